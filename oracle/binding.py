@@ -1,0 +1,398 @@
+"""ctypes bindings for the CPU oracle (liboracle.so) and, where it was built,
+the reference harness (oracle/_ref/libdsmcref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import struct
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+P = ctypes.c_void_p
+D = ctypes.c_double
+L = ctypes.c_long
+
+_lib = None
+_ref = None
+
+
+def _ptr(a):
+    return a.ctypes.data_as(P)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built; run __graft_entry__.build()")
+        _lib = ctypes.CDLL(path)
+        _lib.or_system_create.restype = P
+        _lib.or_system_seed.restype = L
+        _lib.or_system_num.restype = L
+        _lib.or_system_moments.restype = L
+        _lib.or_u01.restype = D
+        _lib.or_time_until_collision.restype = D
+        _lib.or_voxel_index_halo.restype = L
+        _lib.or_cell_index_from_position.restype = L
+        _lib.or_geom_porosity.restype = ctypes.c_float
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libdsmcref.so"))
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libdsmcref.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/_ref/libdsmcref.so is not built (needs the reference sources)")
+        _ref = ctypes.CDLL(path)
+        _ref.ref_sys_create.restype = P
+        _ref.ref_sys_num.restype = L
+        _ref.ref_sys_log_size.restype = L
+        for f in ("ref_sys_num", "ref_sys_get_state", "ref_sys_set_state", "ref_sys_move_all",
+                  "ref_sys_step", "ref_sys_counters", "ref_sys_log_size", "ref_sys_log",
+                  "ref_sys_log_clear", "ref_sys_num_active", "ref_sys_cells", "ref_sys_cell_volumes"):
+            getattr(_ref, f).argtypes = None
+    return _ref
+
+
+# ---------------------------------------------------------------- RNG
+def ran1_sequence(seed, nu, ng, use_ref=False):
+    u = np.zeros(max(nu, 1))
+    g = np.zeros(max(ng, 1))
+    if use_ref:
+        ref().ref_ran1_sequence(L(seed), nu, _ptr(u), ng, _ptr(g))
+    else:
+        lib().or_ran1_sequence(L(seed), nu, _ptr(u), ng, _ptr(g))
+    return u[:nu], g[:ng]
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
+    k = np.ascontiguousarray(key, dtype=np.uint32).reshape(2)
+    out = np.zeros_like(c)
+    lib().or_philox_batch(len(c), _ptr(c), _ptr(k), _ptr(out))
+    return out
+
+
+def u01(x):
+    x = np.asarray(x, dtype=np.uint64)
+    return (((x >> 9) * 2 + 1).astype(np.float64)) * (1.0 / 16777216.0)
+
+
+# ---------------------------------------------------------------- units
+def units():
+    """unitconverter.cpp:3-21 via the oracle."""
+    class U(ctypes.Structure):
+        _fields_ = [(n, D) for n in ("m0", "L0", "E0", "kb", "t0", "T0", "v0", "a0", "number_density0")]
+    u = U()
+    lib().or_units(ctypes.byref(u))
+    return {n: getattr(u, n) for n, _ in U._fields_}
+
+
+# ---------------------------------------------------------------- geometry
+def packed_spheres(n, radius_vox, porosity, seed, nna=1, use_ref=False, spheres_type=1,
+                   spheres_num=0, inverted=True, prevent_overlap=False):
+    nv = n ** 3
+    t = np.zeros(nv, np.uint8)
+    nr = np.zeros(3 * nv, np.float32)
+    a = np.zeros(3 * nv, np.float32)
+    b = np.zeros(3 * nv, np.float32)
+    por = np.zeros(1, np.float32)
+    fn = ref().ref_packed_spheres if use_ref else lib().or_geom_packed_spheres
+    fn(n, n, n, spheres_type, spheres_num, D(porosity), int(seed), D(radius_vox / n), int(inverted),
+       int(prevent_overlap), nna, _ptr(t), _ptr(nr), _ptr(a), _ptr(b), _ptr(por))
+    return {"n": (n, n, n), "types": t, "normals": nr, "t1": a, "t2": b, "porosity": float(por[0])}
+
+
+def classify(shape, mask, nna=1):
+    nx, ny, nz = shape
+    nv = nx * ny * nz
+    t = np.ascontiguousarray(mask, dtype=np.uint8).reshape(nv).copy()
+    nr = np.zeros(3 * nv, np.float32)
+    a = np.zeros(3 * nv, np.float32)
+    b = np.zeros(3 * nv, np.float32)
+    lib().or_geom_classify(nx, ny, nz, nna, _ptr(t), _ptr(nr), _ptr(a), _ptr(b))
+    por = float(lib().or_geom_porosity(nx, ny, nz, _ptr(t)))
+    return {"n": (nx, ny, nz), "types": t, "normals": nr, "t1": a, "t2": b, "porosity": por}
+
+
+def poly_spheres(shape, rmin, rmax, porosity, seed, nna=1):
+    nx, ny, nz = shape
+    m = np.zeros(nx * ny * nz, np.uint8)
+    ns = lib().or_geom_poly_spheres_mask(nx, ny, nz, D(rmin), D(rmax), D(porosity),
+                                         ctypes.c_uint64(seed), _ptr(m))
+    w = classify(shape, m, nna)
+    w["spheres"] = ns
+    w["mask"] = m
+    return w
+
+
+def write_world_file(folder, world):
+    """One-rank halo world file (dsmc_io.cpp:183-220 layout), written by the
+    reference's own writer when available, else by numpy tiling."""
+    os.makedirs(folder, exist_ok=True)
+    nx, ny, nz = world["n"]
+    if ref_available():
+        ref().ref_save_world(folder.encode(), nx, ny, nz, _ptr(world["types"]), _ptr(world["normals"]),
+                             _ptr(world["t1"]), _ptr(world["t2"]))
+        return os.path.join(folder, "0000.bin")
+    t = world["types"].reshape(nx, ny, nz)
+    idx = lambda n: (np.arange(-n, 2 * n) + 10 * n) % n
+    ii, jj, kk = idx(nx), idx(ny), idx(nz)
+    ht = t[np.ix_(ii, jj, kk)]
+    vec = lambda a: a.reshape(nx, ny, nz, 3)[np.ix_(ii, jj, kk)].reshape(-1)
+    path = os.path.join(folder, "0000.bin")
+    with open(path, "wb") as f:
+        f.write(struct.pack("<ff", world["porosity"], world["porosity"]))
+        f.write(struct.pack("<6I", nx, ny, nz, 3 * nx, 3 * ny, 3 * nz))
+        f.write(ht.tobytes())
+        for key in ("normals", "t1", "t2"):
+            f.write(vec(world[key]).astype(np.float32).tobytes())
+    return path
+
+
+# ---------------------------------------------------------------- system
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("L", D * 3), ("cells", ctypes.c_int * 3),
+        ("dt", D), ("mass", D), ("temperature", D), ("wall_temperature", D),
+        ("gravity", D), ("flow_direction", ctypes.c_int), ("density", D), ("diam", D),
+        ("atoms_per_molecule", D), ("wall_model", ctypes.c_int), ("rng_mode", ctypes.c_int),
+        ("seed", L), ("key", ctypes.c_uint64), ("porosity", ctypes.c_float),
+    ]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("steps", L), ("t", D), ("collisions", ctypes.c_ulong), ("wall_collisions", ctypes.c_ulong),
+                ("reseeds", ctypes.c_ulong), ("errors", ctypes.c_ulong), ("count_periodic", L * 3),
+                ("candidates", ctypes.c_ulong)]
+
+
+RAN1, PHILOX = 0, 1
+THERMAL, SPECULAR = 0, 1
+
+
+def params_from_settings(ini, world, rng_mode=PHILOX, key=None):
+    """Reduced-unit parameters exactly as System::initialize derives them from
+    the settings (system.cpp:253-266, unitconverter.cpp)."""
+    u = units()
+    p = Params()
+    p.nx, p.ny, p.nz = world["n"]
+    p.L[0], p.L[1], p.L[2] = float(ini["Lx"]), float(ini["Ly"]), float(ini["Lz"])
+    p.cells[0], p.cells[1], p.cells[2] = int(ini["cells_x"]), int(ini["cells_y"]), int(ini["cells_z"])
+    p.dt = float(ini["dt"])
+    p.mass = float(ini["mass"])
+    p.temperature = float(ini["temperature"]) / u["T0"]
+    p.wall_temperature = float(ini["wall_temperature"]) / u["T0"]
+    p.gravity = float(ini["gravity"])
+    p.flow_direction = int(ini["flow_direction"])
+    p.density = float(ini["density"]) / u["number_density0"]
+    p.diam = float(ini["diam"])
+    p.atoms_per_molecule = float(int(float(ini["atoms_per_molecule"])))
+    p.wall_model = SPECULAR if ini["surface_interaction_model"] == "specular" else THERMAL
+    p.rng_mode = rng_mode
+    p.seed = int(float(ini["seed"]))
+    p.key = int(key if key is not None else int(float(ini["seed"])))
+    p.porosity = world["porosity"]
+    return p
+
+
+class OracleSystem:
+    def __init__(self, params: Params, world):
+        self.lib = lib()
+        self.p = params
+        self.h = self.lib.or_system_create(ctypes.byref(params), _ptr(world["types"]),
+                                           _ptr(world["normals"]), _ptr(world["t1"]), _ptr(world["t2"]))
+        self.ncells = self.lib.or_system_num_cells(P(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.or_system_destroy(P(self.h))
+            self.h = None
+
+    def seed(self):
+        return self.lib.or_system_seed(P(self.h))
+
+    def load_state(self, rv6):
+        rv6 = np.ascontiguousarray(rv6, dtype=np.float64)
+        return self.lib.or_system_load_state(P(self.h), _ptr(rv6), L(len(rv6)))
+
+    @property
+    def n(self):
+        return self.lib.or_system_num(P(self.h))
+
+    def state(self):
+        out = np.zeros((self.n, 6))
+        self.lib.or_system_get_state(P(self.h), _ptr(out))
+        return out
+
+    def set_state(self, rv6):
+        rv6 = np.ascontiguousarray(rv6, dtype=np.float64)
+        assert rv6.shape == (self.n, 6)
+        self.lib.or_system_set_state_inplace(P(self.h), _ptr(rv6))
+
+    def step(self, phases=7):
+        self.lib.or_system_step(P(self.h), phases)
+
+    def counters(self):
+        c = Counters()
+        self.lib.or_system_counters(P(self.h), ctypes.byref(c))
+        return {"steps": c.steps, "t": c.t, "collisions": c.collisions, "wall_collisions": c.wall_collisions,
+                "reseeds": c.reseeds, "errors": c.errors, "count_periodic": list(c.count_periodic),
+                "candidates": c.candidates}
+
+    def cells(self):
+        n = self.ncells
+        out = {k: np.zeros(n) for k in ("volume", "coeff", "vr_max", "rest")}
+        active = np.zeros(n, np.int32)
+        count = np.zeros(n, np.int32)
+        self.lib.or_system_cell_data(P(self.h), _ptr(out["volume"]), _ptr(out["coeff"]), _ptr(out["vr_max"]),
+                                     _ptr(out["rest"]), _ptr(active), _ptr(count))
+        out["active"] = active.astype(bool)
+        out["count"] = count
+        return out
+
+    def set_cell_state(self, vr_max=None, rest=None):
+        a = None if vr_max is None else np.ascontiguousarray(vr_max, dtype=np.float64)
+        b = None if rest is None else np.ascontiguousarray(rest, dtype=np.float64)
+        self.lib.or_system_set_cell_state(P(self.h), None if a is None else _ptr(a), None if b is None else _ptr(b))
+
+    def collide_stats(self):
+        pairs = np.zeros(self.ncells, np.int32)
+        acc = np.zeros(self.ncells, np.int32)
+        self.lib.or_system_cell_collide_stats(P(self.h), _ptr(pairs), _ptr(acc))
+        return pairs, acc
+
+    def last_perm(self):
+        p = np.zeros(self.n, np.int32)
+        self.lib.or_system_last_perm(P(self.h), _ptr(p))
+        return p
+
+    def last_wall(self):
+        nh = np.zeros(self.n, np.int32)
+        lv = np.zeros(self.n, np.int32)
+        self.lib.or_system_last_wall(P(self.h), _ptr(nh), _ptr(lv))
+        return nh, lv
+
+    def last_margin(self):
+        m = np.zeros(self.n)
+        self.lib.or_system_last_margin(P(self.h), _ptr(m))
+        return m
+
+    def sample(self):
+        self.lib.or_system_sample(P(self.h))
+
+    def moments(self):
+        m = np.zeros((self.ncells, 7))
+        ns = self.lib.or_system_moments(P(self.h), _ptr(m))
+        return m, ns
+
+    def totals(self):
+        t = np.zeros(4)
+        self.lib.or_system_totals(P(self.h), _ptr(t))
+        return t
+
+
+def stable_counting_sort(keys, ncell):
+    k = np.ascontiguousarray(keys, dtype=np.int32)
+    perm = np.zeros(len(k), np.int32)
+    start = np.zeros(ncell + 1, np.int32)
+    lib().or_stable_counting_sort(L(len(k)), _ptr(k), ncell, _ptr(perm), _ptr(start))
+    return perm, start
+
+
+# ---------------------------------------------------------------- reference
+def write_ini(path, ini):
+    with open(path, "w") as f:
+        for k, v in ini.items():
+            f.write(f"{k} = {v}\n")
+
+
+class RefSystem:
+    """The compiled reference System driven through oracle/_ref."""
+
+    def __init__(self, ini, world, workdir=None):
+        self.r = ref()
+        self.tmp = None
+        if workdir is None:
+            self.tmp = tempfile.TemporaryDirectory(prefix="dsmcref_")
+            workdir = self.tmp.name
+        os.makedirs(os.path.join(workdir, "statistics", "time"), exist_ok=True)
+        write_world_file(os.path.join(workdir, "world"), world)
+        ini = dict(ini)
+        ini["world"] = "world"
+        write_ini(os.path.join(workdir, "dsmc.ini"), ini)
+        self.h = self.r.ref_sys_create(workdir.encode(), b"dsmc.ini")
+        if not self.h:
+            raise RuntimeError("reference System failed to initialise")
+
+    @property
+    def n(self):
+        return self.r.ref_sys_num(P(self.h))
+
+    def state(self):
+        out = np.zeros((self.n, 6))
+        self.r.ref_sys_get_state(P(self.h), _ptr(out))
+        return out
+
+    def set_state(self, rv6):
+        rv6 = np.ascontiguousarray(rv6, dtype=np.float64)
+        assert rv6.shape == (self.n, 6)
+        self.r.ref_sys_set_state(P(self.h), _ptr(rv6))
+
+    def move_all(self, dt):
+        self.r.ref_sys_move_all(P(self.h), D(dt))
+
+    def step(self):
+        self.r.ref_sys_step(P(self.h))
+
+    def counters(self):
+        o = np.zeros(8)
+        self.r.ref_sys_counters(P(self.h), _ptr(o))
+        return {"steps": int(o[0]), "t": o[1], "collisions": int(o[2]), "wall_calls": int(o[3]),
+                "count_periodic": [int(x) for x in o[4:7]], "thermal_collisions": int(o[7])}
+
+    def wall_log(self, clear=True):
+        n = self.r.ref_sys_log_size(P(self.h))
+        vox = np.zeros(max(n, 1), np.int64)
+        part = np.zeros(max(n, 1), np.int64)
+        vin = np.zeros(max(3 * n, 1))
+        vout = np.zeros(max(3 * n, 1))
+        self.r.ref_sys_log(P(self.h), _ptr(vox), _ptr(part), _ptr(vin), _ptr(vout))
+        if clear:
+            self.r.ref_sys_log_clear(P(self.h))
+        return vox[:n], part[:n], vin[:3 * n].reshape(n, 3), vout[:3 * n].reshape(n, 3)
+
+    def cells(self, members=False):
+        na = self.r.ref_sys_num_active(P(self.h))
+        idx = np.zeros(na, np.int32)
+        cnt = np.zeros(na, np.int32)
+        vr = np.zeros(na)
+        rest = np.zeros(na)
+        mem = np.zeros(max(self.n, 1), np.int32) if members else None
+        self.r.ref_sys_cells(P(self.h), _ptr(idx), _ptr(cnt), _ptr(vr), _ptr(rest),
+                             _ptr(mem) if members else None)
+        out = {"index": idx, "count": cnt, "vr_max": vr, "rest": rest}
+        if members:
+            out["members"] = mem
+        return out
+
+    def cell_volumes(self):
+        na = self.r.ref_sys_num_active(P(self.h))
+        idx = np.zeros(na, np.int32)
+        vol = np.zeros(na)
+        coeff = np.zeros(na)
+        self.r.ref_sys_cell_volumes(P(self.h), _ptr(idx), _ptr(vol), _ptr(coeff))
+        return idx, vol, coeff

@@ -1,0 +1,1180 @@
+/*
+ * dsmc_oracle.c -- FP64 CPU oracle for the DSMC timestep (TEST INFRASTRUCTURE).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this library; the product (master_amd/libdsmc.so) never does.
+ *
+ * Compile with -ffp-contract=off: every expression below is evaluated in the
+ * same order and precision (float vs double) as the reference statement it
+ * restates, so that in OR_RNG_RAN1 mode results are bit-identical to the
+ * compiled reference.
+ *
+ * Pinning (tests/test_oracle_*.py):
+ *   or_ran1_next / or_ran1_gauss ......... known-answer sequences captured from
+ *                                          the compiled reference (golden).
+ *   or_philox4x32_10 ..................... Random123 published known-answer
+ *                                          vectors (3 cases).
+ *   or_geom_packed_spheres / classify .... bit-exact vs the reference geometry
+ *                                          tool output (golden, 32^3 and 64^3).
+ *   or_geom_poly_spheres_mask ............ porosity target reached, periodicity
+ *                                          invariant, brute-force sphere test.
+ *   mover + face time + colliders ........ bit-exact vs reference trajectories
+ *                                          under specular walls and vs the
+ *                                          reference's thermal-wall collider log
+ *                                          (golden); face time vs analytic slab
+ *                                          solve on non-degenerate rays.
+ *   cells / NTC / re-bin / seeding ....... whole-step bit-exact replay of the
+ *                                          reference (ran1 mode, golden hashes);
+ *                                          closed-form N and coeff.
+ *   stable counting sort ................. equals numpy stable argsort.
+ *   moments / totals ..................... brute-force numpy sums.
+ *   Philox-mode wall and collide draws ... distributional checks (Rayleigh and
+ *                                          Gaussian wall components, uniform
+ *                                          cos(theta)); the mapping of Philox
+ *                                          words to draws itself has no
+ *                                          reference counterpart.
+ *   or_units ............................. closed forms (t0 = 1.00294e-9 s etc.).
+ */
+#include "dsmc_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+/* ======================================================================
+ * RNG: Numerical Recipes ran1 (random.h:3-11, random.cpp:43-68)
+ * ====================================================================== */
+#define R1_IA 16807
+#define R1_IM 2147483647
+#define R1_AM (1.0 / R1_IM)
+#define R1_IQ 127773
+#define R1_IR 2836
+#define R1_NTAB 32
+#define R1_NDIV (1 + (R1_IM - 1) / R1_NTAB)
+#define R1_EPS 1.2e-7
+#define R1_RNMX (1.0 - R1_EPS)
+
+void or_ran1_seed(or_ran1_t *r, long seed) {   /* random.cpp:8-19 */
+    r->idum = seed;
+    r->iy = 0;
+    memset(r->iv, 0, sizeof(r->iv));
+}
+
+double or_ran1_next(or_ran1_t *r) {            /* random.cpp:43-68 */
+    int j;
+    long k;
+    double temp;
+    if (r->idum <= 0 || !r->iy) {
+        if (-(r->idum) < 1) r->idum = 1;
+        else r->idum = -(r->idum);
+        for (j = R1_NTAB + 7; j >= 0; j--) {
+            k = r->idum / R1_IQ;
+            r->idum = R1_IA * (r->idum - k * R1_IQ) - R1_IR * k;
+            if (r->idum < 0) r->idum += R1_IM;
+            if (j < R1_NTAB) r->iv[j] = r->idum;
+        }
+        r->iy = r->iv[0];
+    }
+    k = r->idum / R1_IQ;
+    r->idum = R1_IA * (r->idum - k * R1_IQ) - R1_IR * k;
+    if (r->idum < 0) r->idum += R1_IM;
+    j = (int)(r->iy / R1_NDIV);
+    r->iy = r->iv[j];
+    r->iv[j] = r->idum;
+    if ((temp = R1_AM * r->iy) > R1_RNMX) return R1_RNMX;
+    return temp;
+}
+
+/* random.cpp:21-25.  The two draws are operands of one product, whose
+ * evaluation order C++ leaves unspecified; the compiled reference (g++ 11,
+ * x86-64) draws the logarithm's uniform first.  The order is pinned by the
+ * golden next_gauss sequence. */
+double or_ran1_gauss(or_ran1_t *r) {
+    double u_log = or_ran1_next(r);
+    double u_cos = or_ran1_next(r);
+    return sqrt(-2.0 * log(1.0 - u_log)) * cos(6.283185307 * u_cos);
+}
+
+void or_ran1_sequence(long seed, int n_uniform, double *u, int n_gauss, double *g) {
+    or_ran1_t r;
+    or_ran1_seed(&r, seed);
+    for (int i = 0; i < n_uniform; i++) u[i] = or_ran1_next(&r);
+    for (int i = 0; i < n_gauss; i++) g[i] = or_ran1_gauss(&r);
+}
+
+/* ======================================================================
+ * RNG: Philox4x32-10 (Salmon et al., SC'11, "Parallel random numbers: as
+ * easy as 1, 2, 3"; constants from the paper / Random123 distribution).
+ * ====================================================================== */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int rnd = 0; rnd < 10; rnd++) {
+        if (rnd > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+void or_philox_batch(int n, const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4) {
+    for (int i = 0; i < n; i++) or_philox4x32_10(ctr4 + 4 * i, key2, out4 + 4 * i);
+}
+
+/* Odd 24-bit numerator: (2m+1) * 2^-24 with m = x >> 9.  Exactly
+ * representable in float and double, never 0 or 1. */
+double or_u01(uint32_t x) {
+    return (double)((x >> 9) * 2u + 1u) * (1.0 / 16777216.0);
+}
+
+/* Counter layout shared with the GPU kernels: ctr = {tag, step, index, sub}. */
+enum { TAG_INIT_POS = 1, TAG_INIT_VEL = 2, TAG_WALL = 3, TAG_RESEED = 4,
+       TAG_SELECT = 5, TAG_SCATTER = 6 };
+
+static void philox_draw(uint64_t key, uint32_t tag, uint32_t step, uint32_t index,
+                        uint32_t sub, uint32_t out[4]) {
+    uint32_t c[4] = {tag, step, index, sub};
+    uint32_t k[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
+    or_philox4x32_10(c, k, out);
+}
+
+/* ======================================================================
+ * Units (unitconverter.cpp:3-21)
+ * ====================================================================== */
+void or_units(or_units_t *u) {
+    u->m0 = 1.66053886e-27;
+    u->L0 = 1e-6;
+    u->E0 = 1.65088e-21;
+    u->kb = 1.3806503e-23;
+    u->t0 = u->L0 * sqrt(u->m0 / u->E0);
+    u->T0 = u->E0 / u->kb;
+    u->v0 = u->L0 / u->t0;
+    u->a0 = u->v0 / u->t0;
+    u->number_density0 = 1.0 / (u->L0 * u->L0 * u->L0);
+}
+
+/* ======================================================================
+ * Geometry (complexgeometry.cpp)
+ * ====================================================================== */
+static void make_periodic(int nx, int ny, int nz, uint8_t *t) {   /* 853-871 */
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+                int gi = nx - i - 1, gj = ny - j - 1, gk = nz - k - 1;
+                if (i < nx / 2) gi = i;
+                if (j < ny / 2) gj = j;
+                if (k < nz / 2) gk = k;
+                t[(long)i * ny * nz + j * nz + k] = t[(long)gi * ny * nz + gj * nz + gk];
+            }
+}
+
+static void calculate_normals(int nx, int ny, int nz, int nna, uint8_t *t, float *nrm) { /* 928-999 */
+    long nv = (long)nx * ny * nz;
+    memset(nrm, 0, 3 * nv * sizeof(float));
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+                int at_least_one_wall_neighbor = 0;
+                int all_neighbors_are_walls = 1;
+                long idx = (long)i * ny * nz + j * nz + k;
+                float *n = nrm + 3 * idx;
+                for (int di = -1; di <= 1; di++)
+                    for (int dj = -1; dj <= 1; dj++)
+                        for (int dk = -1; dk <= 1; dk++) {
+                            long idx2 = (long)((i + di + nx) % nx) * ny * nz +
+                                        ((j + dj + ny) % ny) * nz + ((k + dk + nz) % nz);
+                            if (t[idx2] > 0) at_least_one_wall_neighbor = 1;
+                            if (t[idx2] == 0) all_neighbors_are_walls = 0;
+                            n[0] -= (float)(t[idx2] * di);
+                            n[1] -= (float)(t[idx2] * dj);
+                            n[2] -= (float)(t[idx2] * dk);
+                        }
+                double norm_squared = (double)(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                if (norm_squared > 0) {
+                    n[0] = 0; n[1] = 0; n[2] = 0;
+                    for (int di = -nna; di <= nna; di++)
+                        for (int dj = -nna; dj <= nna; dj++)
+                            for (int dk = -nna; dk <= nna; dk++) {
+                                long idx2 = (long)((i + di + nx) % nx) * ny * nz +
+                                            ((j + dj + ny) % ny) * nz + ((k + dk + nz) % nz);
+                                n[0] -= (float)(t[idx2] * di);
+                                n[1] -= (float)(t[idx2] * dj);
+                                n[2] -= (float)(t[idx2] * dk);
+                            }
+                }
+                /* <cmath> resolves sqrt(float) to the float overload */
+                double norm = (double)sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                if (norm > 0) {
+                    n[0] = (float)(n[0] / norm);
+                    n[1] = (float)(n[1] / norm);
+                    n[2] = (float)(n[2] / norm);
+                } else if (!at_least_one_wall_neighbor || !all_neighbors_are_walls) {
+                    t[idx] = 0;
+                }
+            }
+}
+
+static void calculate_tangents(int nx, int ny, int nz, const float *nrm, float *t1, float *t2) { /* 1001-1046 */
+    or_ran1_t rnd;
+    or_ran1_seed(&rnd, -1);
+    long nv = (long)nx * ny * nz;
+    for (long idx = 0; idx < nv; idx++) {
+        const float *n = nrm + 3 * idx;
+        float *a = t1 + 3 * idx, *b = t2 + 3 * idx;
+        a[0] = (float)or_ran1_next(&rnd);
+        a[1] = (float)or_ran1_next(&rnd);
+        a[2] = (float)or_ran1_next(&rnd);
+        double dot_product = (double)(n[0] * a[0] + n[1] * a[1] + n[2] * a[2]);
+        a[0] = (float)((double)a[0] - (double)n[0] * dot_product);
+        a[1] = (float)((double)a[1] - (double)n[1] * dot_product);
+        a[2] = (float)((double)a[2] - (double)n[2] * dot_product);
+        /* <cmath> resolves sqrt(float) to the float overload */
+        double norm = (double)sqrtf(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        if (norm > 0) {
+            a[0] = (float)(a[0] / norm);
+            a[1] = (float)(a[1] / norm);
+            a[2] = (float)(a[2] / norm);
+        }
+        b[0] = a[1] * n[2] - a[2] * n[1];
+        b[1] = a[2] * n[0] - a[0] * n[2];
+        b[2] = a[0] * n[1] - a[1] * n[0];
+        norm = (double)sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        if (norm > 0) {
+            b[0] = (float)(b[0] / norm);
+            b[1] = (float)(b[1] / norm);
+            b[2] = (float)(b[2] / norm);
+        }
+    }
+}
+
+static void find_boundary_points(int nx, int ny, int nz, uint8_t *t, const float *nrm) { /* 1048-1063 */
+    long nv = (long)nx * ny * nz;
+    for (long idx = 0; idx < nv; idx++) {
+        const float *n = nrm + 3 * idx;
+        double normal_norm = (double)(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (t[idx] > 0 && normal_norm > 0) t[idx] = 2;
+    }
+}
+
+void or_geom_classify(int nx, int ny, int nz, int nna, uint8_t *types,
+                      float *normals, float *t1, float *t2) {        /* 196-201 */
+    calculate_normals(nx, ny, nz, nna, types, normals);
+    calculate_tangents(nx, ny, nz, normals, t1, t2);
+    find_boundary_points(nx, ny, nz, types, normals);
+}
+
+float or_geom_porosity(int nx, int ny, int nz, const uint8_t *types) {   /* 173-179 */
+    long nv = (long)nx * ny * nz;
+    int sum = 0;
+    for (long i = 0; i < nv; i++) sum += types[i] == OR_EMPTY;
+    return (float)((double)sum / nv);
+}
+
+int or_geom_packed_spheres(int nx, int ny, int nz, int spheres_type, int spheres_num,
+                           double wanted_porosity, int seed_in, double radius, int inverted,
+                           int prevent_overlap, int nna,
+                           uint8_t *types, float *normals, float *t1, float *t2,
+                           float *porosity_out) {                        /* 401-513 */
+    int seed = -abs(seed_in);
+    double radius_squared = radius * radius;
+    float voxel_size_x = (float)(1.0 / nx);
+    float voxel_size_y = (float)(1.0 / ny);
+    float voxel_size_z = (float)(1.0 / nz);
+    int nx_half = nx / 2, ny_half = ny / 2, nz_half = nz / 2;
+    long nv = (long)nx * ny * nz;
+    for (long i = 0; i < nv; i++) types[i] = (uint8_t)(!inverted);
+    or_ran1_t rnd;
+    or_ran1_seed(&rnd, seed);
+    int nspheres = 0, cap = 1024;
+    double *pos = (double *)malloc(3 * sizeof(double) * cap);
+    int is_finished = 0;
+    while (!is_finished) {
+        double x0 = or_ran1_next(&rnd) * 0.5;
+        double y0 = or_ran1_next(&rnd) * 0.5;
+        double z0 = or_ran1_next(&rnd) * 0.5;
+        int inside_another_sphere = 1;
+        while (prevent_overlap && inside_another_sphere) {
+            x0 = or_ran1_next(&rnd) * 0.5;
+            y0 = or_ran1_next(&rnd) * 0.5;
+            z0 = or_ran1_next(&rnd) * 0.5;
+            inside_another_sphere = 0;
+            for (int o = 0; o < nspheres; o++) {
+                double dx = pos[3 * o] - x0, dy = pos[3 * o + 1] - y0, dz = pos[3 * o + 2] - z0;
+                double dr = sqrt(dx * dx + dy * dy + dz * dz);
+                if (dr < 2 * radius) { inside_another_sphere = 1; break; }
+            }
+        }
+        if (nspheres == cap) { cap *= 2; pos = (double *)realloc(pos, 3 * sizeof(double) * cap); }
+        pos[3 * nspheres] = x0; pos[3 * nspheres + 1] = y0; pos[3 * nspheres + 2] = z0;
+        nspheres++;
+        int empty_voxels = 0;
+        for (int i = 0; i < nx_half; i++)
+            for (int j = 0; j < ny_half; j++)
+                for (int k = 0; k < nz_half; k++) {
+                    double x = i / (double)nx + voxel_size_x / 2.0;
+                    double y = j / (double)ny + voxel_size_y / 2.0;
+                    double z = k / (double)nz + voxel_size_z / 2.0;
+                    double dx = x - x0, dy = y - y0, dz = z - z0;
+                    double dr2 = dx * dx + dy * dy + dz * dz;
+                    long index = (long)i * ny * nz + j * nz + k;
+                    if (dr2 < radius_squared) types[index] = (uint8_t)inverted;
+                    empty_voxels += types[index] == OR_EMPTY;
+                }
+        if (spheres_type == 0) {
+            is_finished = nspheres >= spheres_num;
+        } else {
+            double porosity = (double)empty_voxels / (nx_half * ny_half * nz_half);
+            is_finished = inverted ? (porosity < wanted_porosity) : (porosity > wanted_porosity);
+        }
+    }
+    free(pos);
+    make_periodic(nx, ny, nz, types);
+    or_geom_classify(nx, ny, nz, nna, types, normals, t1, t2);
+    if (porosity_out) *porosity_out = or_geom_porosity(nx, ny, nz, types);
+    return nspheres;
+}
+
+/* Build-only polydisperse generator: spheres with centres uniform in the
+ * periodic box and radii log-uniform in [rmin, rmax] voxels, added until the
+ * empty fraction drops below wanted_porosity.  Sphere s draws its centre and
+ * radius from Philox(ctr={7, 0, s, 0}, key=seed). */
+int or_geom_poly_spheres_mask(int nx, int ny, int nz, double rmin, double rmax,
+                              double wanted_porosity, uint64_t seed, uint8_t *mask) {
+    long nv = (long)nx * ny * nz;
+    memset(mask, 0, nv);
+    long empty = nv;
+    int s = 0;
+    double lr0 = log(rmin), lr1 = log(rmax);
+    while ((double)empty / (double)nv >= wanted_porosity) {
+        uint32_t o[4];
+        philox_draw(seed, 7u, 0u, (uint32_t)s, 0u, o);
+        double cx = or_u01(o[0]) * nx, cy = or_u01(o[1]) * ny, cz = or_u01(o[2]) * nz;
+        double rad = exp(lr0 + (lr1 - lr0) * or_u01(o[3]));
+        double r2 = rad * rad;
+        int i0 = (int)floor(cx - rad) - 1, i1 = (int)ceil(cx + rad) + 1;
+        int j0 = (int)floor(cy - rad) - 1, j1 = (int)ceil(cy + rad) + 1;
+        int k0 = (int)floor(cz - rad) - 1, k1 = (int)ceil(cz + rad) + 1;
+        for (int i = i0; i <= i1; i++) {
+            double dx = (i + 0.5) - cx;
+            int ii = ((i % nx) + nx) % nx;
+            for (int j = j0; j <= j1; j++) {
+                double dy = (j + 0.5) - cy;
+                int jj = ((j % ny) + ny) % ny;
+                for (int k = k0; k <= k1; k++) {
+                    double dz = (k + 0.5) - cz;
+                    int kk = ((k % nz) + nz) % nz;
+                    if (dx * dx + dy * dy + dz * dz < r2) {
+                        long idx = (long)ii * ny * nz + (long)jj * nz + kk;
+                        if (!mask[idx]) { mask[idx] = 1; empty--; }
+                    }
+                }
+            }
+        }
+        s++;
+    }
+    return s;
+}
+
+/* ======================================================================
+ * System
+ * ====================================================================== */
+struct or_system {
+    or_params_t p;
+    int nx, ny, nz;
+    long nv;
+    uint8_t *type;
+    float *nrm, *tg1, *tg2;
+    /* grid.cpp:17-47 */
+    float k0, k1, k2;       /* one_over_length * global_n (float members) */
+    float c0, c1, c2;       /* constant_0..2                              */
+    double vsize[3];
+    double one_over_length[3];
+    double length[3];
+    or_ran1_t rnd;
+    /* particles */
+    long n, cap;
+    double *r, *v;
+    /* cells */
+    int ncells;
+    int *cell_map;          /* global -> active index or -1            */
+    int nactive;
+    int *active_global;     /* active -> global (creation order)       */
+    double *volume, *coeff, *vr_max, *rest;  /* per global cell        */
+    int *pixels, *total_pixels;
+    /* ran1 mode: reference membership lists */
+    int **mol, *nmol, *capmol;
+    int *mol_index_in_cell, *mol_cell_index;
+    /* counter mode: sorted ranges */
+    int32_t *start;
+    int32_t *perm;
+    /* last-phase records */
+    int32_t *nhits, *lasthit;
+    double *margin;
+    int *cell_pairs, *cell_acc;
+    /* counters */
+    or_counters_t cnt;
+    double sqrt_wall_temp_over_mass;
+    double sqrt_temp_over_mass;
+    /* moments */
+    double *mom;
+    long nsamples;
+    long cur_index;         /* particle index of the move in progress  */
+    int cur_hit;
+};
+
+static inline long halo_index(const or_system *s, int i, int j, int k) {
+    return (long)i * (3 * s->ny) * (3 * s->nz) + (long)j * (3 * s->nz) + k;
+}
+
+/* The reference reads a halo-tiled copy (3x per axis) of the periodic grid
+ * (complexgeometry.cpp:251-281, dsmc_io.cpp:183-220); for one rank the halo
+ * voxel (i,j,k) is global voxel (i mod n, j mod n, k mod n). */
+static inline long global_from_halo(const or_system *s, long hidx, int *ok) {
+    long hy = 3L * s->ny, hz = 3L * s->nz;
+    long i = hidx / (hy * hz), j = (hidx / hz) % hy, k = hidx % hz;
+    if (hidx < 0 || i >= 3L * s->nx) { *ok = 0; return 0; }
+    *ok = 1;
+    return (i % s->nx) * s->ny * s->nz + (j % s->ny) * s->nz + (k % s->nz);
+}
+
+static inline int type_halo(or_system *s, long hidx) {
+    int ok;
+    long g = global_from_halo(s, hidx, &ok);
+    if (!ok) { s->cnt.errors++; return OR_WALL; }
+    return s->type[g];
+}
+
+long or_voxel_index_halo(const or_system *s, const double *r) {   /* grid.h:139-145 */
+    const int i = (int)(r[0] * s->k0 + s->c0);
+    const int j = (int)(r[1] * s->k1 + s->c1);
+    const int k = (int)(r[2] * s->k2 + s->c2);
+    return halo_index(s, i, j, k);
+}
+
+/* grid.cpp:61-71: get_voxel(vector r, index) with double constants */
+static long voxel_index_double(const or_system *s, const double *r) {
+    int i = (int)((r[0] - 0.0) * s->one_over_length[0] * s->nx + (double)s->nx);
+    int j = (int)((r[1] - 0.0) * s->one_over_length[1] * s->ny + (double)s->ny);
+    int k = (int)((r[2] - 0.0) * s->one_over_length[2] * s->nz + (double)s->nz);
+    return halo_index(s, i, j, k);
+}
+
+long or_cell_index_from_position(const or_system *s, const double *r) {  /* system.h:122-132 */
+    int i = (int)(r[0] * s->one_over_length[0] * s->p.cells[0]);
+    int j = (int)(r[1] * s->one_over_length[1] * s->p.cells[1]);
+    int k = (int)(r[2] * s->one_over_length[2] * s->p.cells[2]);
+    return (long)i * s->p.cells[1] * s->p.cells[2] + (long)j * s->p.cells[2] + k;
+}
+
+/* ---------- CVector restatement (cvector.h) for the face-time solve ---------- */
+typedef struct { double x, y, z; } vec3;
+static inline vec3 v3(double x, double y, double z) { vec3 a = {x, y, z}; return a; }
+static inline vec3 vadd(vec3 self, vec3 a) { return v3(a.x + self.x, a.y + self.y, a.z + self.z); }
+static inline vec3 vsub(vec3 self, vec3 a) { return v3(self.x - a.x, self.y - a.y, self.z - a.z); }
+static inline vec3 vscale(vec3 self, double s) { return v3(self.x * s, self.y * s, self.z * s); }
+static inline double vdot(vec3 self, vec3 o) { return (double)(self.x * o.x + self.y * o.y + self.z * o.z); }
+static inline vec3 vcross(vec3 self, vec3 o) {
+    return v3(o.y * self.z - o.z * self.y, o.z * self.x - o.x * self.z, o.x * self.y - o.y * self.x);
+}
+static inline double plane_time(vec3 r, vec3 v, vec3 p, vec3 n) {   /* grid.h:17-19 */
+    return vdot(vsub(p, r), n) / vdot(v, n);
+}
+static inline int same_side(vec3 p1, vec3 p2, vec3 a, vec3 b) {    /* grid.h:21-26 */
+    vec3 cp1 = vcross(vsub(b, a), vsub(p1, a));
+    vec3 cp2 = vcross(vsub(b, a), vsub(p2, a));
+    return vdot(cp1, cp2) >= 0;
+}
+static inline int within_square(vec3 p1, vec3 p2, vec3 p3, vec3 p4, vec3 pt) { /* grid.h:28-34 */
+    vec3 center = vscale(vadd(vadd(vadd(p1, p2), p3), p4), 0.25);
+    return same_side(pt, center, p1, p2) && same_side(pt, center, p2, p3) &&
+           same_side(pt, center, p3, p4) && same_side(pt, center, p4, p1);
+}
+
+#define OR_SENTINEL 1e9
+
+/* grid.h:82-137.  Returns the reference value; *no_face is set when no face
+ * was hit (the reference's unguarded 1e9 sentinel). */
+static double time_until_collision(const or_system *s, const double *r, const double *v,
+                                   long voxel_index, int *no_face, double *edge_margin) {
+    vec3 rv = v3(r[0] - 0.0, r[1] - 0.0, r[2] - 0.0);
+    vec3 vv = v3(v[0], v[1], v[2]);
+    double tuc = OR_SENTINEL;
+    long hy = 3L * s->ny, hz = 3L * s->nz;
+    int i = (int)(voxel_index / (hy * hz));
+    int j = (int)((voxel_index / hz) % hy);
+    int k = (int)(voxel_index % hz);
+    i -= s->nx; j -= s->ny; k -= s->nz;
+    const double *vs = s->vsize;
+    vec3 pl[8];
+    pl[0] = v3(i * vs[0], j * vs[1], k * vs[2]);
+    pl[1] = v3((i + 1) * vs[0], j * vs[1], k * vs[2]);
+    pl[2] = v3(i * vs[0], j * vs[1], (k + 1) * vs[2]);
+    pl[3] = v3((i + 1) * vs[0], j * vs[1], (k + 1) * vs[2]);
+    pl[4] = v3(i * vs[0], (j + 1) * vs[1], k * vs[2]);
+    pl[5] = v3((i + 1) * vs[0], (j + 1) * vs[1], k * vs[2]);
+    pl[6] = v3(i * vs[0], (j + 1) * vs[1], (k + 1) * vs[2]);
+    pl[7] = v3((i + 1) * vs[0], (j + 1) * vs[1], (k + 1) * vs[2]);
+    vec3 un[6] = {v3(-1, 0, 0), v3(1, 0, 0), v3(0, -1, 0), v3(0, 1, 0), v3(0, 0, -1), v3(0, 0, 1)};
+    double tf[6];
+    tf[0] = plane_time(rv, vv, pl[2], un[0]);
+    tf[1] = plane_time(rv, vv, pl[1], un[1]);
+    tf[2] = plane_time(rv, vv, pl[0], un[2]);
+    tf[3] = plane_time(rv, vv, pl[4], un[3]);
+    tf[4] = plane_time(rv, vv, pl[0], un[4]);
+    tf[5] = plane_time(rv, vv, pl[2], un[5]);
+    int hit[6];
+    hit[0] = within_square(pl[4], pl[0], pl[2], pl[6], vadd(rv, vscale(vv, tf[0])));
+    hit[1] = within_square(pl[1], pl[5], pl[7], pl[3], vadd(rv, vscale(vv, tf[1])));
+    hit[2] = within_square(pl[0], pl[1], pl[3], pl[2], vadd(rv, vscale(vv, tf[2])));
+    hit[3] = within_square(pl[5], pl[4], pl[6], pl[7], vadd(rv, vscale(vv, tf[3])));
+    hit[4] = within_square(pl[4], pl[5], pl[1], pl[0], vadd(rv, vscale(vv, tf[4])));
+    hit[5] = within_square(pl[2], pl[3], pl[7], pl[6], vadd(rv, vscale(vv, tf[5])));
+    for (int f = 0; f < 6; f++)
+        if (tf[f] > 0 && tf[f] < tuc && hit[f] && !isnan(tf[f])) tuc = tf[f];
+    if (edge_margin) {
+        /* distance (voxel units) of each candidate hit point to its face's
+         * edges: near zero means the inside test is FP-ambiguous */
+        double m = 1e300;
+        for (int f = 0; f < 6; f++) {
+            if (!(tf[f] > 0) || isnan(tf[f]) || isinf(tf[f])) continue;
+            vec3 ph = vadd(rv, vscale(vv, tf[f]));
+            double c[3] = {ph.x / vs[0] - i, ph.y / vs[1] - j, ph.z / vs[2] - k};
+            for (int a = 0; a < 3; a++) {
+                if (a == f / 2) continue;
+                double d0 = fabs(c[a]), d1 = fabs(c[a] - 1.0);
+                if (d0 < m) m = d0;
+                if (d1 < m) m = d1;
+            }
+        }
+        *edge_margin = m;
+    }
+    *no_face = (tuc == OR_SENTINEL);
+    if (tuc < 1e-5) return 0;
+    return tuc - 1e-7;
+}
+
+double or_time_until_collision(const or_system *s, const double *r, const double *v, long vi) {
+    int nf;
+    return time_until_collision(s, r, v, vi, &nf, NULL);
+}
+
+/* ---------- seeding / re-seeding (system.cpp:384-396) ---------- */
+static void find_position(or_system *s, long index, int tag_step) {
+    double *r = s->r + 3 * index;
+    if (s->p.rng_mode == OR_RNG_RAN1) {
+        for (;;) {
+            r[0] = 0.0 + s->length[0] * or_ran1_next(&s->rnd);
+            r[1] = 0.0 + s->length[1] * or_ran1_next(&s->rnd);
+            r[2] = 0.0 + s->length[2] * or_ran1_next(&s->rnd);
+            int did_collide = type_halo(s, voxel_index_double(s, r)) >= OR_WALL;
+            int is_inside = r[0] >= 0 && r[0] < s->length[0] && r[1] >= 0 &&
+                            r[1] < s->length[1] && r[2] >= 0 && r[2] < s->length[2];
+            if (!did_collide && is_inside) return;
+        }
+    } else {
+        /* Counter mode: pick a voxel uniformly by integer index and reject
+         * solid ones, then a uniform offset inside it (same distribution as
+         * the reference's rejection in the box). */
+        uint32_t tag = tag_step < 0 ? TAG_INIT_POS : TAG_RESEED;
+        uint32_t step = tag_step < 0 ? 0u : (uint32_t)tag_step;
+        for (uint32_t attempt = 0;; attempt++) {
+            uint32_t o[4];
+            philox_draw(s->p.key, tag, step, (uint32_t)index, attempt, o);
+            uint32_t lin = (uint32_t)(((uint64_t)o[0] * (uint64_t)s->nv) >> 32);
+            if (s->type[lin] != OR_EMPTY) {
+                if (attempt > 100000) { s->cnt.errors++; return; }
+                continue;
+            }
+            int gi = (int)(lin / ((uint32_t)s->ny * s->nz));
+            int gj = (int)((lin / s->nz) % s->ny);
+            int gk = (int)(lin % s->nz);
+            r[0] = ((double)gi + or_u01(o[1])) * s->vsize[0];
+            r[1] = ((double)gj + or_u01(o[2])) * s->vsize[1];
+            r[2] = ((double)gk + or_u01(o[3])) * s->vsize[2];
+            return;
+        }
+    }
+}
+
+/* ---------- wall models ---------- */
+static void wall_collide(or_system *s, double *v, long cvi_halo, long index, int hit) {
+    int ok;
+    long g = global_from_halo(s, cvi_halo, &ok);
+    const float *n = s->nrm + 3 * g, *a = s->tg1 + 3 * g, *b = s->tg2 + 3 * g;
+    if (s->p.wall_model == OR_WALL_SPECULAR) {                   /* colliderspecular.cpp:8-32 */
+        float n_x = n[0], n_y = n[1], n_z = n[2];
+        float t1_x = a[0], t1_y = a[1], t1_z = a[2];
+        float t2_x = b[0], t2_y = b[1], t2_z = b[2];
+        double v_normal = n_x * v[0] + n_y * v[1] + n_z * v[2];
+        double v_tangent1 = t1_x * v[0] + t1_y * v[1] + t1_z * v[2];
+        double v_tangent2 = t2_x * v[0] + t2_y * v[1] + t2_z * v[2];
+        v[0] = -v_normal * n_x + v_tangent1 * t1_x + v_tangent2 * t2_x;
+        v[1] = -v_normal * n_y + v_tangent1 * t1_y + v_tangent2 * t2_y;
+        v[2] = -v_normal * n_z + v_tangent1 * t1_z + v_tangent2 * t2_z;
+    } else {                                                      /* colliderthermal.cpp:13-46 */
+        double sw = s->sqrt_wall_temp_over_mass;
+        double v_normal, v_tangent1, v_tangent2;
+        if (s->p.rng_mode == OR_RNG_RAN1) {
+            v_normal = sw * sqrt(-2 * log(or_ran1_next(&s->rnd)));
+            v_tangent1 = sw * or_ran1_gauss(&s->rnd);
+            v_tangent2 = sw * or_ran1_gauss(&s->rnd);
+        } else {
+            uint32_t o[4];
+            philox_draw(s->p.key, TAG_WALL, (uint32_t)s->cnt.steps, (uint32_t)index, (uint32_t)hit, o);
+            double rr = sqrt(-2.0 * log(or_u01(o[1])));
+            double th = 6.283185307179586 * or_u01(o[2]);
+            v_normal = sw * sqrt(-2.0 * log(or_u01(o[0])));
+            v_tangent1 = sw * (rr * cos(th));
+            v_tangent2 = sw * (rr * sin(th));
+        }
+        float n_x = n[0], n_y = n[1], n_z = n[2];
+        float t1_x = a[0], t1_y = a[1], t1_z = a[2];
+        float t2_x = b[0], t2_y = b[1], t2_z = b[2];
+        v[0] = v_normal * n_x + v_tangent1 * t1_x + v_tangent2 * t2_x;
+        v[1] = v_normal * n_y + v_tangent1 * t1_y + v_tangent2 * t2_y;
+        v[2] = v_normal * n_z + v_tangent1 * t1_z + v_tangent2 * t2_z;
+    }
+    s->cnt.wall_collisions++;
+    s->nhits[index]++;
+    s->lasthit[index] = (int32_t)g;
+}
+
+static inline double face_margin(const or_system *s, const double *r) {
+    double m = 1e300;
+    for (int a = 0; a < 3; a++) {
+        double x = r[a] * (a == 0 ? s->nx : a == 1 ? s->ny : s->nz) * s->one_over_length[a];
+        double f = x - floor(x);
+        double d = f < 1.0 - f ? f : 1.0 - f;
+        if (d < m) m = d;
+    }
+    return m;
+}
+
+static inline void do_move(double *r, const double *v, double dt) {   /* moleculemover.cpp:37-41 */
+    r[0] += v[0] * dt;
+    r[1] += v[1] * dt;
+    r[2] += v[2] * dt;
+}
+
+/* moleculemover.cpp:50-126, with the self-recursion unrolled into a loop
+ * (the recursive call is always the last action of the caller). */
+static void move_molecule(or_system *s, long mi, double dt_in) {
+    double dt = dt_in;
+    double *r = s->r + 3 * mi, *v = s->v + 3 * mi;
+    double *mg = &s->margin[mi];
+    int step_tag = (int)s->cnt.steps;
+    for (int depth = 0;; depth++) {
+        double tau = dt;
+        long idx = or_voxel_index_halo(s, r);
+        if (type_halo(s, idx) != OR_EMPTY) {
+            find_position(s, mi, step_tag);
+            s->cnt.reseeds++;
+            return;
+        }
+        do_move(r, v, tau);
+        idx = or_voxel_index_halo(s, r);
+        { double m = face_margin(s, r); if (m < *mg) *mg = m; }
+        int count = 0;
+        if (type_halo(s, idx) >= OR_WALL) {
+            int resumed = 0;
+            while (type_halo(s, idx) != OR_BOUNDARY) {
+                if (++count > 100) {
+                    find_position(s, mi, step_tag);
+                    s->cnt.reseeds++;
+                    return;
+                }
+                if (type_halo(s, idx) == OR_WALL) {
+                    tau /= 2;
+                    do_move(r, v, -tau);
+                    idx = or_voxel_index_halo(s, r);
+                    { double m = face_margin(s, r); if (m < *mg) *mg = m; }
+                } else {
+                    dt -= tau;
+                    resumed = 1;
+                    break;
+                }
+            }
+            if (resumed) {
+                if (dt > 1e-5 && depth < 10) continue;
+                return;
+            }
+            long cvi = idx;
+            count = 0;
+            while (type_halo(s, idx) == OR_BOUNDARY) {
+                if (++count > 100) {
+                    find_position(s, mi, step_tag);
+                    s->cnt.reseeds++;
+                    return;
+                }
+                cvi = idx;
+                do_move(r, v, -tau);
+                int no_face = 0;
+                double em = 1e300;
+                tau = time_until_collision(s, r, v, cvi, &no_face, &em);
+                if (em < *mg) *mg = em;
+                if (no_face) {
+                    /* The reference would jump 1e9 time units and index out
+                     * of its grid; the oracle (and the GPU) count an error
+                     * and re-seed instead. */
+                    s->cnt.errors++;
+                    find_position(s, mi, step_tag);
+                    s->cnt.reseeds++;
+                    return;
+                }
+                do_move(r, v, tau);
+                idx = or_voxel_index_halo(s, r);
+                { double m = face_margin(s, r); if (m < *mg) *mg = m; }
+            }
+            dt -= tau;
+            wall_collide(s, v, cvi, mi, s->cur_hit++);
+        } else {
+            dt = 0;
+        }
+        if (!(dt > 1e-5 && depth < 10)) return;
+    }
+}
+
+static inline void periodic(or_system *s, double *r) {   /* moleculemover.h:29-38 */
+    for (int a = 0; a < 3; a++) {
+        if (r[a] >= s->length[a]) { r[a] -= s->length[a]; s->cnt.count_periodic[a]++; }
+        else if (r[a] < 0) { r[a] += s->length[a]; s->cnt.count_periodic[a]--; }
+    }
+}
+
+/* ---------- cells (system.cpp:432-490, cell.cpp:27-40) ---------- */
+static void cell_add(or_system *s, int gcell, int m) {     /* cell.cpp:113-124 */
+    if (s->nmol[gcell] + 1 > s->capmol[gcell]) {
+        int nc = s->capmol[gcell] ? 2 * s->capmol[gcell] : 100;
+        s->mol[gcell] = (int *)realloc(s->mol[gcell], sizeof(int) * nc);
+        s->capmol[gcell] = nc;
+    }
+    s->mol[gcell][s->nmol[gcell]] = m;
+    s->mol_index_in_cell[m] = s->nmol[gcell];
+    s->mol_cell_index[m] = gcell;
+    s->nmol[gcell]++;
+}
+
+static void cell_remove(or_system *s, int gcell, int m) {  /* cell.cpp:126-140 */
+    if (s->nmol[gcell] > 1) {
+        int cur = s->mol_index_in_cell[m];
+        int last = s->mol[gcell][s->nmol[gcell] - 1];
+        s->mol[gcell][cur] = last;
+        s->mol_index_in_cell[last] = s->mol_index_in_cell[m];
+    }
+    s->nmol[gcell]--;
+}
+
+static void setup_cells(or_system *s) {
+    int cx = s->p.cells[0], cy = s->p.cells[1], cz = s->p.cells[2];
+    int nc = cx * cy * cz;
+    s->ncells = nc;
+    int *tmp_map = (int *)malloc(sizeof(int) * nc);
+    int *order = (int *)malloc(sizeof(int) * nc);
+    int ntmp = 0;
+    for (int c = 0; c < nc; c++) tmp_map[c] = -1;
+    for (int k = 0; k < s->nz; k++) {
+        int c_z = (int)((float)(k + 0) / s->nz * cz);
+        for (int j = 0; j < s->ny; j++) {
+            int c_y = (int)((float)(j + 0) / s->ny * cy);
+            for (int i = 0; i < s->nx; i++) {
+                int c_x = (int)((float)(i + 0) / s->nx * cx);
+                int ci = c_x * cy * cz + c_y * cz + c_z;
+                if (tmp_map[ci] == -1) { tmp_map[ci] = ntmp; order[ntmp++] = ci; }
+                s->total_pixels[ci]++;
+                s->pixels[ci] += s->type[(long)i * s->ny * s->nz + (long)j * s->nz + k] < OR_WALL;
+            }
+        }
+    }
+    double system_volume = s->length[0] * s->length[1] * s->length[2];
+    s->nactive = 0;
+    for (int c = 0; c < nc; c++) s->cell_map[c] = -1;
+    double vr0 = 3 * sqrt(s->p.temperature);                     /* system.cpp:262,477 */
+    for (int t = 0; t < ntmp; t++) {
+        int ci = order[t];
+        s->vr_max[ci] = vr0;
+        s->rest[ci] = 0;
+        if (s->pixels[ci] == 0) { s->volume[ci] = 0; s->coeff[ci] = 0; continue; }
+        s->volume[ci] = system_volume / (nc) * (float)s->pixels[ci] / s->total_pixels[ci];
+        s->coeff[ci] = 0.5 * s->p.atoms_per_molecule * M_PI * s->p.diam * s->p.diam * s->p.dt / s->volume[ci];
+        if (s->volume[ci] > 0) {
+            s->cell_map[ci] = s->nactive;
+            s->active_global[s->nactive++] = ci;
+        }
+    }
+    free(tmp_map);
+    free(order);
+}
+
+or_system *or_system_create(const or_params_t *p, const uint8_t *types,
+                            const float *normals, const float *t1, const float *t2) {
+    or_system *s = (or_system *)calloc(1, sizeof(or_system));
+    s->p = *p;
+    s->nx = p->nx; s->ny = p->ny; s->nz = p->nz;
+    s->nv = (long)s->nx * s->ny * s->nz;
+    s->type = (uint8_t *)malloc(s->nv);
+    memcpy(s->type, types, s->nv);
+    s->nrm = (float *)malloc(12 * s->nv);
+    s->tg1 = (float *)malloc(12 * s->nv);
+    s->tg2 = (float *)malloc(12 * s->nv);
+    memcpy(s->nrm, normals, 12 * s->nv);
+    memcpy(s->tg1, t1, 12 * s->nv);
+    memcpy(s->tg2, t2, 12 * s->nv);
+    for (int a = 0; a < 3; a++) {
+        s->length[a] = p->L[a];
+        s->one_over_length[a] = 1.0 / p->L[a];
+    }
+    int gn[3] = {s->nx, s->ny, s->nz};
+    for (int a = 0; a < 3; a++) s->vsize[a] = s->length[a] / gn[a];
+    /* grid.cpp:41-46 (origin 0, voxel_origin = n) */
+    s->c0 = (float)(-0.0 * s->one_over_length[0] * gn[0] + (double)gn[0]);
+    s->c1 = (float)(-0.0 * s->one_over_length[1] * gn[1] + (double)gn[1]);
+    s->c2 = (float)(-0.0 * s->one_over_length[2] * gn[2] + (double)gn[2]);
+    s->k0 = (float)(s->one_over_length[0] * gn[0]);
+    s->k1 = (float)(s->one_over_length[1] * gn[1]);
+    s->k2 = (float)(s->one_over_length[2] * gn[2]);
+    /* system.cpp:492-497 */
+    long seed = p->seed;
+    seed = -labs(seed + 1242461L * (0 + 1));
+    or_ran1_seed(&s->rnd, seed);
+    s->sqrt_wall_temp_over_mass = sqrt(p->wall_temperature / p->mass);
+    s->sqrt_temp_over_mass = sqrt(p->temperature / p->mass);
+    int nc = p->cells[0] * p->cells[1] * p->cells[2];
+    s->cell_map = (int *)calloc(nc, sizeof(int));
+    s->active_global = (int *)calloc(nc, sizeof(int));
+    s->volume = (double *)calloc(nc, sizeof(double));
+    s->coeff = (double *)calloc(nc, sizeof(double));
+    s->vr_max = (double *)calloc(nc, sizeof(double));
+    s->rest = (double *)calloc(nc, sizeof(double));
+    s->pixels = (int *)calloc(nc, sizeof(int));
+    s->total_pixels = (int *)calloc(nc, sizeof(int));
+    s->mol = (int **)calloc(nc, sizeof(int *));
+    s->nmol = (int *)calloc(nc, sizeof(int));
+    s->capmol = (int *)calloc(nc, sizeof(int));
+    s->start = (int32_t *)calloc(nc + 1, sizeof(int32_t));
+    s->cell_pairs = (int *)calloc(nc, sizeof(int));
+    s->cell_acc = (int *)calloc(nc, sizeof(int));
+    s->mom = (double *)calloc(7 * (size_t)nc, sizeof(double));
+    setup_cells(s);
+    return s;
+}
+
+void or_system_destroy(or_system *s) {
+    if (!s) return;
+    free(s->type); free(s->nrm); free(s->tg1); free(s->tg2);
+    free(s->r); free(s->v);
+    free(s->cell_map); free(s->active_global); free(s->volume); free(s->coeff);
+    free(s->vr_max); free(s->rest); free(s->pixels); free(s->total_pixels);
+    for (int c = 0; c < s->ncells; c++) free(s->mol[c]);
+    free(s->mol); free(s->nmol); free(s->capmol);
+    free(s->mol_index_in_cell); free(s->mol_cell_index);
+    free(s->start); free(s->perm); free(s->nhits); free(s->lasthit); free(s->margin);
+    free(s->cell_pairs); free(s->cell_acc); free(s->mom);
+    free(s);
+}
+
+static void alloc_particles(or_system *s, long n) {
+    s->n = n;
+    s->cap = n > 0 ? n : 1;
+    s->r = (double *)realloc(s->r, 3 * sizeof(double) * s->cap);
+    s->v = (double *)realloc(s->v, 3 * sizeof(double) * s->cap);
+    s->mol_index_in_cell = (int *)realloc(s->mol_index_in_cell, sizeof(int) * s->cap);
+    s->mol_cell_index = (int *)realloc(s->mol_cell_index, sizeof(int) * s->cap);
+    s->perm = (int32_t *)realloc(s->perm, sizeof(int32_t) * s->cap);
+    s->nhits = (int32_t *)realloc(s->nhits, sizeof(int32_t) * s->cap);
+    s->lasthit = (int32_t *)realloc(s->lasthit, sizeof(int32_t) * s->cap);
+    s->margin = (double *)realloc(s->margin, sizeof(double) * s->cap);
+    for (long i = 0; i < n; i++) { s->nhits[i] = 0; s->lasthit[i] = -1; s->margin[i] = 1e300; s->perm[i] = (int32_t)i; }
+}
+
+void or_stable_counting_sort(long n, const int32_t *key, int ncell, int32_t *perm, int32_t *start) {
+    int32_t *cnt = (int32_t *)calloc(ncell + 1, sizeof(int32_t));
+    for (long i = 0; i < n; i++) cnt[key[i]]++;                 /* sortParticles.m:11-38 */
+    start[0] = 0;
+    for (int c = 0; c < ncell; c++) start[c + 1] = start[c] + cnt[c];   /* 41-45 */
+    for (int c = 0; c < ncell; c++) cnt[c] = 0;
+    for (long i = 0; i < n; i++) {                              /* 48-63 */
+        int c = key[i];
+        perm[start[c] + cnt[c]] = (int32_t)i;
+        cnt[c]++;
+    }
+    free(cnt);
+}
+
+/* Counter mode: physically reorder particles by cell (stable). */
+static int sort_by_cell(or_system *s) {
+    int32_t *key = (int32_t *)malloc(sizeof(int32_t) * (s->n ? s->n : 1));
+    int bad = 0;
+    for (long i = 0; i < s->n; i++) {
+        long c = or_cell_index_from_position(s, s->r + 3 * i);
+        if (c < 0 || c >= s->ncells || s->cell_map[c] < 0) { bad++; c = 0; }
+        key[i] = (int32_t)c;
+    }
+    or_stable_counting_sort(s->n, key, s->ncells, s->perm, s->start);
+    double *r2 = (double *)malloc(3 * sizeof(double) * (s->n ? s->n : 1));
+    double *v2 = (double *)malloc(3 * sizeof(double) * (s->n ? s->n : 1));
+    for (long i = 0; i < s->n; i++) {
+        long o = s->perm[i];
+        memcpy(r2 + 3 * i, s->r + 3 * o, 3 * sizeof(double));
+        memcpy(v2 + 3 * i, s->v + 3 * o, 3 * sizeof(double));
+    }
+    memcpy(s->r, r2, 3 * sizeof(double) * s->n);
+    memcpy(s->v, v2, 3 * sizeof(double) * s->n);
+    free(r2); free(v2); free(key);
+    for (int c = 0; c < s->ncells; c++) s->nmol[c] = s->start[c + 1] - s->start[c];
+    s->cnt.errors += bad;
+    return bad;
+}
+
+long or_system_seed(or_system *s) {                       /* system.cpp:313-333, 398-430 */
+    double volume = s->length[0] * s->length[1] * s->length[2];
+    double volume_per_cpu = volume / 1;
+    volume = volume_per_cpu * s->p.porosity;
+    long n = (long)(s->p.density * volume / s->p.atoms_per_molecule);
+    alloc_particles(s, n);
+    double stm = s->sqrt_temp_over_mass;
+    for (long i = 0; i < n; i++) {
+        double *v = s->v + 3 * i;
+        if (s->p.rng_mode == OR_RNG_RAN1) {
+            v[0] = or_ran1_gauss(&s->rnd) * stm;
+            v[1] = or_ran1_gauss(&s->rnd) * stm;
+            v[2] = or_ran1_gauss(&s->rnd) * stm;
+        } else {
+            uint32_t o[4];
+            philox_draw(s->p.key, TAG_INIT_VEL, 0u, (uint32_t)i, 0u, o);
+            double r01 = sqrt(-2.0 * log(or_u01(o[0])));
+            double th01 = 6.283185307179586 * or_u01(o[1]);
+            double r2 = sqrt(-2.0 * log(or_u01(o[2])));
+            double th2 = 6.283185307179586 * or_u01(o[3]);
+            v[0] = (r01 * cos(th01)) * stm;
+            v[1] = (r01 * sin(th01)) * stm;
+            v[2] = (r2 * cos(th2)) * stm;
+        }
+        find_position(s, i, -1);
+        if (s->p.rng_mode == OR_RNG_RAN1) {
+            long c = or_cell_index_from_position(s, s->r + 3 * i);
+            if (c < 0 || c >= s->ncells || s->cell_map[c] < 0) { s->cnt.errors++; continue; }
+            cell_add(s, (int)c, (int)i);
+        }
+    }
+    if (s->p.rng_mode == OR_RNG_PHILOX) sort_by_cell(s);
+    return n;
+}
+
+int or_system_load_state(or_system *s, const double *rv6, long n) {   /* dsmc_io.cpp:121-159 */
+    alloc_particles(s, n);
+    for (int c = 0; c < s->ncells; c++) s->nmol[c] = 0;
+    for (long i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++) { s->r[3 * i + a] = rv6[6 * i + a]; s->v[3 * i + a] = rv6[6 * i + 3 + a]; }
+    }
+    if (s->p.rng_mode == OR_RNG_PHILOX) return sort_by_cell(s);
+    for (long i = 0; i < n; i++) {
+        long c = or_cell_index_from_position(s, s->r + 3 * i);
+        if (c < 0 || c >= s->ncells || s->cell_map[c] < 0) { s->cnt.errors++; continue; }
+        cell_add(s, (int)c, (int)i);
+    }
+    return 0;
+}
+
+long or_system_num(const or_system *s) { return s->n; }
+
+void or_system_get_state(const or_system *s, double *rv6) {
+    for (long i = 0; i < s->n; i++)
+        for (int a = 0; a < 3; a++) { rv6[6 * i + a] = s->r[3 * i + a]; rv6[6 * i + 3 + a] = s->v[3 * i + a]; }
+}
+
+void or_system_set_state_inplace(or_system *s, const double *rv6) {
+    for (long i = 0; i < s->n; i++)
+        for (int a = 0; a < 3; a++) { s->r[3 * i + a] = rv6[6 * i + a]; s->v[3 * i + a] = rv6[6 * i + 3 + a]; }
+}
+
+/* ---------- NTC (cell.cpp:42-111) ---------- */
+static void collide_pair(or_system *s, double *v0, double *v1, double v_rel, double u_cos, double u_phi) {
+    double vcmx = 0.5 * (v0[0] + v1[0]);                     /* cell.cpp:53-73 */
+    double vcmy = 0.5 * (v0[1] + v1[1]);
+    double vcmz = 0.5 * (v0[2] + v1[2]);
+    double cos_th = 1.0 - 2.0 * u_cos;
+    double sin_th = sqrt(1.0 - cos_th * cos_th);
+    double phi = 2 * M_PI * u_phi;
+    double vrelx = v_rel * cos_th;
+    double vrely = v_rel * sin_th * cos(phi);
+    double vrelz = v_rel * sin_th * sin(phi);
+    v0[0] = vcmx + 0.5 * vrelx;
+    v0[1] = vcmy + 0.5 * vrely;
+    v0[2] = vcmz + 0.5 * vrelz;
+    v1[0] = vcmx - 0.5 * vrelx;
+    v1[1] = vcmy - 0.5 * vrely;
+    v1[2] = vcmz - 0.5 * vrelz;
+    (void)s;
+}
+
+static int collide_cell(or_system *s, int g) {
+    int n = s->nmol[g];
+    /* prepare (cell.cpp:42-51) */
+    double select = s->coeff[g] * n * (n - 1) * s->vr_max[g] + s->rest[g];
+    int pairs = (int)round(select);
+    s->rest[g] = select - pairs;
+    s->cell_pairs[g] = pairs;
+    s->cell_acc[g] = 0;
+    if (n < 2) return 0;
+    double crm = s->vr_max[g];
+    int collisions = 0;
+    for (int isel = 0; isel < pairs; isel++) {
+        int ip0, ip1;
+        uint32_t o[4];
+        double u_acc;
+        if (s->p.rng_mode == OR_RNG_RAN1) {
+            ip0 = (int)(or_ran1_next(&s->rnd) * n);
+            ip1 = ((int)(ip0 + 1 + or_ran1_next(&s->rnd) * (n - 1))) % n;
+            ip0 = s->mol[g][ip0];
+            ip1 = s->mol[g][ip1];
+        } else {
+            philox_draw(s->p.key, TAG_SELECT, (uint32_t)s->cnt.steps, (uint32_t)g, (uint32_t)isel, o);
+            int a = (int)(((uint64_t)o[0] * (uint64_t)n) >> 32);
+            int b = (a + 1 + (int)(((uint64_t)o[1] * (uint64_t)(n - 1)) >> 32)) % n;
+            ip0 = s->start[g] + a;
+            ip1 = s->start[g] + b;
+        }
+        double *v0 = s->v + 3 * (long)ip0, *v1 = s->v + 3 * (long)ip1;
+        double v_rel = sqrt(pow(v0[0] - v1[0], 2) + pow(v0[1] - v1[1], 2) + pow(v0[2] - v1[2], 2));
+        if (v_rel > crm) crm = v_rel;
+        s->cnt.candidates++;
+        if (s->p.rng_mode == OR_RNG_RAN1) u_acc = or_ran1_next(&s->rnd);
+        else u_acc = or_u01(o[2]);
+        if (v_rel > u_acc * s->vr_max[g]) {
+            collisions++;
+            double uc, up;
+            if (s->p.rng_mode == OR_RNG_RAN1) {
+                uc = or_ran1_next(&s->rnd);
+                up = or_ran1_next(&s->rnd);
+            } else {
+                uint32_t q[4];
+                philox_draw(s->p.key, TAG_SCATTER, (uint32_t)s->cnt.steps, (uint32_t)g, (uint32_t)isel, q);
+                uc = or_u01(q[0]);
+                up = or_u01(q[1]);
+            }
+            collide_pair(s, v0, v1, v_rel, uc, up);
+        }
+    }
+    s->vr_max[g] = crm;
+    s->cell_acc[g] = collisions;
+    return collisions;
+}
+
+int or_system_step(or_system *s, int phase_mask) {        /* system.cpp:22-47 */
+    s->cnt.steps += 1;
+    s->cnt.t += s->p.dt;
+    for (long i = 0; i < s->n; i++) { s->nhits[i] = 0; s->lasthit[i] = -1; s->margin[i] = 1e300; }
+    if (phase_mask & 1) {
+        /* accelerate (133-144) */
+        if (!(s->p.flow_direction < 0 || s->p.gravity == 0)) {
+            double gdt = s->p.gravity * s->p.dt;
+            for (long i = 0; i < s->n; i++) s->v[3 * i + s->p.flow_direction] += gdt;
+        }
+        /* move (107-120) */
+        for (long i = 0; i < s->n; i++) { s->cur_index = i; s->cur_hit = 0; move_molecule(s, i, s->p.dt); }
+        for (long i = 0; i < s->n; i++) periodic(s, s->r + 3 * i);
+    }
+    if (phase_mask & 2) {
+        if (s->p.rng_mode == OR_RNG_PHILOX) {
+            sort_by_cell(s);
+        } else {                                            /* 146-157 */
+            for (long i = 0; i < s->n; i++) {
+                int oldc = s->mol_cell_index[i];
+                long newc = or_cell_index_from_position(s, s->r + 3 * i);
+                if (newc < 0 || newc >= s->ncells || s->cell_map[newc] < 0) { s->cnt.errors++; continue; }
+                if (oldc != newc) {
+                    cell_remove(s, oldc, (int)i);
+                    cell_add(s, (int)newc, (int)i);
+                }
+            }
+        }
+    }
+    if (phase_mask & 4) {                                   /* 122-131 */
+        for (int a = 0; a < s->nactive; a++) s->cnt.collisions += collide_cell(s, s->active_global[a]);
+    }
+    return 0;
+}
+
+void or_system_counters(const or_system *s, or_counters_t *c) { *c = s->cnt; }
+
+int or_system_num_cells(const or_system *s) { return s->ncells; }
+
+void or_system_cell_data(const or_system *s, double *volume, double *coeff,
+                         double *vr_max, double *rest, int *active, int *count) {
+    for (int c = 0; c < s->ncells; c++) {
+        if (volume) volume[c] = s->volume[c];
+        if (coeff) coeff[c] = s->coeff[c];
+        if (vr_max) vr_max[c] = s->vr_max[c];
+        if (rest) rest[c] = s->rest[c];
+        if (active) active[c] = s->cell_map[c] >= 0;
+        if (count) count[c] = s->nmol[c];
+    }
+}
+
+void or_system_set_cell_state(or_system *s, const double *vr_max, const double *rest) {
+    for (int c = 0; c < s->ncells; c++) {
+        if (vr_max) s->vr_max[c] = vr_max[c];
+        if (rest) s->rest[c] = rest[c];
+    }
+}
+
+void or_system_cell_collide_stats(const or_system *s, int *pairs, int *accepted) {
+    for (int c = 0; c < s->ncells; c++) {
+        if (pairs) pairs[c] = s->cell_pairs[c];
+        if (accepted) accepted[c] = s->cell_acc[c];
+    }
+}
+
+void or_system_last_perm(const or_system *s, int32_t *perm) {
+    memcpy(perm, s->perm, sizeof(int32_t) * s->n);
+}
+
+void or_system_last_wall(const or_system *s, int32_t *nhits, int32_t *last_voxel) {
+    if (nhits) memcpy(nhits, s->nhits, sizeof(int32_t) * s->n);
+    if (last_voxel) memcpy(last_voxel, s->lasthit, sizeof(int32_t) * s->n);
+}
+
+void or_system_last_margin(const or_system *s, double *margin) {
+    memcpy(margin, s->margin, sizeof(double) * s->n);
+}
+
+/* Per-cell moments (matlab/3d/calculatePressure.m:11-32 sums |v|^2 per cell;
+ * here the components are kept separately, in FP64). */
+void or_system_sample(or_system *s) {
+    for (long i = 0; i < s->n; i++) {
+        long c = or_cell_index_from_position(s, s->r + 3 * i);
+        if (c < 0 || c >= s->ncells) continue;
+        const double *v = s->v + 3 * i;
+        double *m = s->mom + 7 * c;
+        m[0] += 1.0;
+        m[1] += v[0]; m[2] += v[1]; m[3] += v[2];
+        m[4] += v[0] * v[0]; m[5] += v[1] * v[1]; m[6] += v[2] * v[2];
+    }
+    s->nsamples++;
+}
+
+long or_system_moments(const or_system *s, double *mom7) {
+    memcpy(mom7, s->mom, sizeof(double) * 7 * (size_t)s->ncells);
+    return s->nsamples;
+}
+
+void or_system_reset_moments(or_system *s) {
+    memset(s->mom, 0, sizeof(double) * 7 * (size_t)s->ncells);
+    s->nsamples = 0;
+}
+
+void or_system_totals(const or_system *s, double *tot4) {
+    double t[4] = {0, 0, 0, 0};
+    for (long i = 0; i < s->n; i++) {
+        const double *v = s->v + 3 * i;
+        t[0] += v[0]; t[1] += v[1]; t[2] += v[2];
+        t[3] += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    }
+    memcpy(tot4, t, sizeof(t));
+}

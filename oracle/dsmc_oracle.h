@@ -1,0 +1,182 @@
+/*
+ * dsmc_oracle.h -- FP64 CPU oracle for the DSMC timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker, never the thing
+ * measured or shipped: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path
+ * (master_amd/libdsmc.so) never links or calls it.
+ *
+ * It restates, in plain C, the reference solver's algorithm for the hot path
+ * (all citations relative to code/base_code/program/dsmc/ unless noted):
+ *   - ran1 / Box-Muller RNG .............. random.h:3-11, random.cpp:8-68
+ *   - packed-sphere geometry, normals,
+ *     tangents, boundary classification .. complexgeometry.cpp:401-513,
+ *                                          853-871, 928-1063, 173-179, 220-318
+ *   - voxel lookup, face-time solve ...... grid.h:17-34, 82-145, grid.cpp:17-71
+ *   - mover, periodic wrap ............... moleculemover.cpp:37-126,
+ *                                          moleculemover.h:29-38
+ *   - diffuse / specular wall models ..... colliderthermal.cpp:13-46,
+ *                                          colliderspecular.cpp:8-32
+ *   - cells, NTC prepare/collide ......... system.cpp:432-490, cell.cpp:27-111
+ *   - seeding ............................ system.cpp:384-430, 241-382
+ *   - incremental re-bin ................. system.cpp:146-157, cell.cpp:113-140
+ *   - stable counting sort ............... matlab/3d/sortParticles.m:11-63
+ *   - per-cell moments ................... matlab/3d/calculatePressure.m:11-32
+ *   - unit conversion .................... unitconverter.cpp:3-66
+ *
+ * Two RNG modes:
+ *   OR_RNG_RAN1    replays the reference's single sequential ran1 stream and its
+ *                  history-dependent cell membership lists, so that whole
+ *                  trajectories can be compared bit-for-bit with the compiled
+ *                  reference (oracle/_ref).
+ *   OR_RNG_PHILOX  uses the counter-based Philox4x32-10 stream keyed exactly as
+ *                  the GPU kernels key it, with particles kept in stable
+ *                  cell-sorted order, so the GPU path can be checked per
+ *                  particle and per candidate pair.
+ *
+ * Pinning (what checks each function) is listed in the header of
+ * dsmc_oracle.c; functions no check pins are marked "parity unpinned" there.
+ */
+#ifndef DSMC_ORACLE_H
+#define DSMC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- RNG ---------------- */
+typedef struct { long idum; long iy; long iv[32]; } or_ran1_t;
+
+void   or_ran1_seed(or_ran1_t *r, long seed);
+double or_ran1_next(or_ran1_t *r);
+double or_ran1_gauss(or_ran1_t *r);
+void   or_ran1_sequence(long seed, int n_uniform, double *u, int n_gauss, double *g);
+
+void   or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double or_u01(uint32_t x);
+void   or_philox_batch(int n, const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4);
+
+/* ---------------- units (unitconverter.cpp:3-66) ---------------- */
+typedef struct {
+    double m0, L0, E0, kb, t0, T0, v0, a0, number_density0;
+} or_units_t;
+void or_units(or_units_t *u);
+
+/* ---------------- geometry ---------------- */
+enum { OR_EMPTY = 0, OR_WALL = 1, OR_BOUNDARY = 2 };
+
+/* Reference packed-sphere generator (complexgeometry.cpp:401-513) followed by
+ * calculate_normals_tangents_and_inner_points (196-201).  Arrays are
+ * z-fastest, size nx*ny*nz (types) and 3*nx*ny*nz (vectors).  radius is in
+ * box units (voxels / n).  Returns the number of spheres placed. */
+int or_geom_packed_spheres(int nx, int ny, int nz, int spheres_type, int spheres_num,
+                           double wanted_porosity, int seed, double radius, int inverted,
+                           int prevent_overlap, int nna,
+                           uint8_t *types, float *normals, float *t1, float *t2,
+                           float *porosity_out);
+
+/* Classification of an arbitrary 0/1 solid mask: calculate_normals(nna),
+ * calculate_tangents() and find_boundary_points() (complexgeometry.cpp:928-1063).
+ * `types` is modified in place (isolated walls cleared, boundary = 2). */
+void or_geom_classify(int nx, int ny, int nz, int nna, uint8_t *types,
+                      float *normals, float *t1, float *t2);
+
+/* Global porosity as written to the world header (complexgeometry.cpp:173-179). */
+float or_geom_porosity(int nx, int ny, int nz, const uint8_t *types);
+
+/* Union of periodic spheres with log-uniform radii (no reference counterpart:
+ * the reference generator supports one radius only).  Counter-RNG keyed by
+ * (seed, sphere index).  Writes a 0/1 mask; classify afterwards. */
+int or_geom_poly_spheres_mask(int nx, int ny, int nz, double rmin_vox, double rmax_vox,
+                              double wanted_porosity, uint64_t seed, uint8_t *mask);
+
+/* ---------------- system ---------------- */
+enum { OR_RNG_RAN1 = 0, OR_RNG_PHILOX = 1 };
+enum { OR_WALL_THERMAL = 0, OR_WALL_SPECULAR = 1 };
+
+typedef struct {
+    int nx, ny, nz;                 /* global voxel counts            */
+    double L[3];                    /* box lengths (reduced units)    */
+    int cells[3];                   /* collision cells per axis       */
+    double dt;                      /* reduced time                   */
+    double mass;                    /* amu                            */
+    double temperature;             /* reduced                        */
+    double wall_temperature;        /* reduced                        */
+    double gravity;                 /* reduced acceleration           */
+    int flow_direction;             /* -1 none, 0..2 axis             */
+    double density;                 /* reduced number density         */
+    double diam;                    /* reduced                        */
+    double atoms_per_molecule;      /* Fn                             */
+    int wall_model;                 /* OR_WALL_*                      */
+    int rng_mode;                   /* OR_RNG_*                       */
+    long seed;                      /* settings->seed                 */
+    uint64_t key;                   /* Philox key (counter mode)      */
+    float porosity;                 /* local porosity from world file */
+} or_params_t;
+
+typedef struct {
+    long steps;
+    double t;
+    unsigned long collisions;       /* accepted NTC pairs (System::collisions) */
+    unsigned long wall_collisions;  /* surface collider calls                  */
+    unsigned long reseeds;          /* find_position re-seeds inside move      */
+    unsigned long errors;           /* face-time sentinel / invalid cell       */
+    long count_periodic[3];
+    unsigned long candidates;       /* NTC candidate pairs examined            */
+} or_counters_t;
+
+typedef struct or_system or_system;
+
+or_system *or_system_create(const or_params_t *p, const uint8_t *types,
+                            const float *normals, const float *t1, const float *t2);
+void   or_system_destroy(or_system *s);
+/* System::setup_molecules equivalent; returns N. */
+long   or_system_seed(or_system *s);
+/* Load a state (reference layout r0,r1,r2,v0,v1,v2 per particle). */
+int    or_system_load_state(or_system *s, const double *rv6, long n);
+long   or_system_num(const or_system *s);
+void   or_system_get_state(const or_system *s, double *rv6);
+void   or_system_set_state_inplace(or_system *s, const double *rv6);
+/* One System::step(); phase_mask bit0 move(+accelerate), bit1 sort/rebin, bit2 collide. */
+int    or_system_step(or_system *s, int phase_mask);
+void   or_system_counters(const or_system *s, or_counters_t *c);
+
+/* Cells: number of cells (global), per-cell data in global (z-fastest) order. */
+int    or_system_num_cells(const or_system *s);
+void   or_system_cell_data(const or_system *s, double *volume, double *coeff,
+                           double *vr_max, double *rest, int *active, int *count);
+void   or_system_set_cell_state(or_system *s, const double *vr_max, const double *rest);
+/* Per-cell pair counts and accepted collisions of the last collide phase. */
+void   or_system_cell_collide_stats(const or_system *s, int *pairs, int *accepted);
+
+/* Counter mode: permutation of the last sort (new index -> old index). */
+void   or_system_last_perm(const or_system *s, int32_t *perm);
+/* Wall-hit log of the last move phase: per particle (old index) the number of
+ * hits and the last hit voxel (global linear z-fastest id, -1 if none). */
+void   or_system_last_wall(const or_system *s, int32_t *nhits, int32_t *last_voxel);
+/* Per particle minimum |distance| (voxel units) of any landing/decision point
+ * of the last move to a voxel face -- marks FP32-ambiguous decisions. */
+void   or_system_last_margin(const or_system *s, double *margin);
+
+/* Sampling (build semantics, a per-cell analogue of statisticalproperty.cpp). */
+void   or_system_sample(or_system *s);
+/* 7 doubles per global cell: n, Sx, Sy, Sz, Sxx, Syy, Szz; plus sample count. */
+long   or_system_moments(const or_system *s, double *mom7);
+void   or_system_reset_moments(or_system *s);
+/* FP64 totals: [0..2] momentum sum, [3] sum |v|^2. */
+void   or_system_totals(const or_system *s, double *tot4);
+
+/* Stand-alone pieces used by unit tests. */
+double or_time_until_collision(const or_system *s, const double *r, const double *v, long voxel_halo_index);
+long   or_voxel_index_halo(const or_system *s, const double *r);
+long   or_cell_index_from_position(const or_system *s, const double *r);
+
+/* Stable counting sort of keys (sortParticles.m:11-63): perm[new]=old,
+ * start[ncell+1]. */
+void   or_stable_counting_sort(long n, const int32_t *key, int ncell, int32_t *perm, int32_t *start);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

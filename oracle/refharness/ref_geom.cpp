@@ -1,0 +1,82 @@
+// Reference-side harness (geometry).  Built only where the reference sources
+// exist, by oracle/Makefile.ref, into oracle/_ref/libdsmcref.so.  It drives
+// the reference's own ComplexGeometry (complexgeometry.cpp:401-513, 853-1063,
+// 220-318) and Random (random.cpp) so that tests/golden/make_golden.py can
+// record their outputs as fixtures.  It is a checker, never product code.
+#include <complexgeometry.h>
+#include <cinifile.h>
+#include <random.h>
+#include <cstring>
+#include <cstdio>
+#include <string>
+
+static void put(CIniFile &ini, const char *name, const std::string &val) {
+    CItem it;
+    it.name = name;
+    it.strval = val;
+    it.dval = strtod(val.c_str(), 0);
+    ini.items.push_back(it);
+}
+
+static std::string num(double x) { char b[64]; snprintf(b, sizeof b, "%.17g", x); return b; }
+
+extern "C" {
+
+void ref_ran1_sequence(long seed, int nu, double *u, int ng, double *g) {
+    Random r(seed, 0, 0);
+    for (int i = 0; i < nu; i++) u[i] = r.next_double();
+    for (int i = 0; i < ng; i++) g[i] = r.next_gauss();
+}
+
+// Runs ComplexGeometry::create_packed_spheres (which ends with make_periodic
+// and calculate_normals_tangents_and_inner_points) and copies its arrays.
+int ref_packed_spheres(int nx, int ny, int nz, int spheres_type, int spheres_num,
+                       double wanted_porosity, int seed, double radius, int inverted,
+                       int prevent_overlap, int nna, unsigned char *types, float *normals,
+                       float *t1, float *t2, float *porosity) {
+    CIniFile ini;
+    put(ini, "num_voxels_x", num(nx));
+    put(ini, "num_voxels_y", num(ny));
+    put(ini, "num_voxels_z", num(nz));
+    put(ini, "spheres_num", num(spheres_num));
+    put(ini, "spheres_type", num(spheres_type));
+    put(ini, "wanted_porosity", num(wanted_porosity));
+    put(ini, "seed", num(seed));
+    put(ini, "radius", num(radius));
+    put(ini, "inverted", inverted ? "true" : "false");
+    put(ini, "prevent_overlap", prevent_overlap ? "true" : "false");
+    put(ini, "number_of_neighbor_averages", num(nna));
+    ComplexGeometry cg;
+    cg.create_packed_spheres(ini);
+    long nv = (long)nx * ny * nz;
+    memcpy(types, cg.vertices_unsigned_char, nv);
+    memcpy(normals, cg.normals, 3 * nv * sizeof(float));
+    memcpy(t1, cg.tangents1, 3 * nv * sizeof(float));
+    memcpy(t2, cg.tangents2, 3 * nv * sizeof(float));
+    cg.calculate_global_porosity();
+    *porosity = cg.global_porosity;
+    return 0;
+}
+
+// Writes the one-rank halo world file with the reference's own writer
+// (complexgeometry.cpp:220-318) from given arrays.
+int ref_save_world(const char *folder, int nx, int ny, int nz, const unsigned char *types,
+                   const float *normals, const float *t1, const float *t2) {
+    CIniFile ini;
+    put(ini, "binary_output_folder", folder);
+    put(ini, "num_processors_x", "1");
+    put(ini, "num_processors_y", "1");
+    put(ini, "num_processors_z", "1");
+    ComplexGeometry cg;
+    cg.allocate(nx, ny, nz);
+    long nv = (long)nx * ny * nz;
+    memcpy(cg.vertices_unsigned_char, types, nv);
+    for (long i = 0; i < nv; i++) cg.vertices[i] = types[i];
+    memcpy(cg.normals, normals, 3 * nv * sizeof(float));
+    memcpy(cg.tangents1, t1, 3 * nv * sizeof(float));
+    memcpy(cg.tangents2, t2, 3 * nv * sizeof(float));
+    cg.save_to_file(ini);
+    return 0;
+}
+
+}
