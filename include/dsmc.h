@@ -1,0 +1,182 @@
+/*
+ * dsmc.h -- C ABI of the MI355X DSMC step library (master_amd/libdsmc.so).
+ *
+ * Drop-in boundary for the reference solver's `System` class
+ * (code/base_code/program/dsmc/system.h:29-120): the loop in main.cpp:57-64
+ * calls System::step() and reads r, v, steps, t, collisions,
+ * mover->count_periodic and surface_collider->num_collisions.  Each entry
+ * point below names the reference interface it replaces.  All functions
+ * return a dsmc_status; none exits the process (the reference exit(1)s on
+ * invalid configurations: system.cpp:293-309, 399-403; moleculemover.cpp
+ * "Error 1/2"; grid.h:130-133).  Host arrays passed in are copied and never
+ * retained.  A handle is not thread-safe; several handles may share a GPU.
+ *
+ * Units are the reference's reduced units (unitconverter.cpp:3-21): length
+ * 1 um, mass 1 amu, energy 1.65088e-21 J, time 1.00294e-9 s.
+ */
+#ifndef DSMC_H
+#define DSMC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    DSMC_OK = 0,
+    DSMC_ERR_INVALID_ARGUMENT = 1,
+    DSMC_ERR_CONFIG = 2,        /* validate_number_of_cells failure (system.cpp:221-239) */
+    DSMC_ERR_CAPACITY = 3,      /* more particles than max_molecules_per_node (system.cpp:399-403) */
+    DSMC_ERR_HIP = 4,           /* a HIP runtime call failed */
+    DSMC_ERR_IO = 5,            /* file could not be read or written */
+    DSMC_ERR_STATE = 6,         /* call out of order (e.g. step before initialize) */
+    DSMC_ERR_NUMERIC = 7        /* device error counters non-zero after a step */
+} dsmc_status;
+
+typedef enum { DSMC_WALL_THERMAL = 0, DSMC_WALL_SPECULAR = 1 } dsmc_wall_model;
+
+/* Settings (settings.h:14-47) after System::initialize's unit conversion
+ * (system.cpp:253-266): every field is in reduced units. */
+typedef struct {
+    double Lx, Ly, Lz;              /* settings Lx, Ly, Lz                   */
+    int cells_x, cells_y, cells_z;  /* settings cells_*                      */
+    double dt;                      /* settings dt                           */
+    double mass;                    /* settings mass (amu)                   */
+    double temperature;             /* temperature_from_SI(settings temperature) */
+    double wall_temperature;        /* temperature_from_SI(wall_temperature) */
+    double gravity;                 /* settings gravity                      */
+    int flow_direction;             /* settings flow_direction (-1: none)    */
+    double density;                 /* number_density_from_SI(settings density) */
+    double diam;                    /* settings diam                         */
+    int atoms_per_molecule;         /* settings atoms_per_molecule (Fn)      */
+    int seed;                       /* settings seed                         */
+    int wall_model;                 /* settings surface_interaction_model    */
+    int64_t max_molecules;          /* settings max_molecules_per_node (0: no cap) */
+} dsmc_settings;
+
+/* Voxel world (the periodic grid a one-rank world file tiles, dsmc_io.cpp:183-220):
+ * types 0 empty, 1 wall, 2 boundary (grid.h:11-15), z-fastest linear index
+ * i*ny*nz + j*nz + k (grid.h:139-145); unit normal and tangents per voxel. */
+typedef struct {
+    uint32_t nx, ny, nz;
+    const uint8_t *types;
+    const float *normals;           /* 3*nx*ny*nz */
+    const float *tangents1;         /* 3*nx*ny*nz */
+    const float *tangents2;         /* 3*nx*ny*nz */
+    float porosity;                 /* local porosity field of the world header */
+} dsmc_world;
+
+typedef struct {
+    int64_t steps;                  /* System::steps                          */
+    double t;                       /* System::t                              */
+    uint64_t collisions;            /* System::collisions                     */
+    uint64_t wall_collisions;       /* surface_collider->num_collisions       */
+    int64_t count_periodic[3];      /* mover->count_periodic                  */
+    uint64_t reseeds;               /* find_position calls inside the mover   */
+    uint64_t errors;                /* face-time sentinel / lost particles    */
+    uint64_t candidates;            /* NTC candidate pairs examined           */
+    int64_t num_molecules;          /* num_molecules_local                    */
+    int64_t max_cell_occupancy;     /* largest cell count seen at last check  */
+    uint64_t overflow_cells;        /* cells processed by the LDS-overflow path */
+} dsmc_counters;
+
+typedef struct dsmc_handle dsmc_handle;
+
+/* --- construction: System::initialize (system.cpp:241-382) ------------- */
+/* Parse a reference dsmc.ini (settings.cpp:3-44, cinifile.h:28-98) and apply
+ * the unit conversion of System::initialize.  world_dir (optional) receives
+ * the `world` key. */
+dsmc_status dsmc_settings_from_ini(const char *path, dsmc_settings *out, char *world_dir, int world_dir_len);
+/* Read a one-rank world file "<dir>/0000.bin" (dsmc_io.cpp:183-220) into
+ * library-owned arrays; release with dsmc_world_free. */
+dsmc_status dsmc_world_read(const char *path, dsmc_world *out);
+void dsmc_world_free(dsmc_world *w);
+
+/* Allocate device state for one system on the current HIP device. */
+dsmc_status dsmc_create(const dsmc_settings *s, const dsmc_world *w, dsmc_handle **out);
+/* Ensemble of independent systems advanced together (no reference
+ * counterpart; one handle, one step call for all members). */
+dsmc_status dsmc_create_ensemble(int n, const dsmc_settings *s, const dsmc_world *w, dsmc_handle **out);
+void dsmc_destroy(dsmc_handle *h);
+const char *dsmc_last_error(const dsmc_handle *h);
+
+/* System::setup_molecules (system.cpp:398-430): N = density*V*phi/Fn
+ * Maxwellian particles uniform in the pore space; returns N via *n_out. */
+dsmc_status dsmc_initialize(dsmc_handle *h, int64_t *n_out);
+/* Member-wise variant for ensembles: n_per_member[i] particles in member i. */
+dsmc_status dsmc_initialize_counts(dsmc_handle *h, const int64_t *n_per_member);
+/* Seed from per-cell fields (density scale, drift, temperature): 5 doubles
+ * per cell (n_rel, ux, uy, uz, T_red) for member `member`; the number of
+ * particles per cell is n_rel * N_mean_per_empty_voxel * empty voxels. */
+dsmc_status dsmc_initialize_fields(dsmc_handle *h, int member, const double *fields5, int64_t *n_out);
+
+/* --- state I/O: DSMC_IO::load/save_state_to_file_binary (dsmc_io.cpp:82-159) */
+/* rv6: per particle r0 r1 r2 v0 v1 v2 (doubles), the reference state layout. */
+dsmc_status dsmc_load_state(dsmc_handle *h, const double *rv6, int64_t n);
+dsmc_status dsmc_get_state(dsmc_handle *h, double *rv6);
+dsmc_status dsmc_num_particles(dsmc_handle *h, int64_t *n);
+dsmc_status dsmc_save_state_file(dsmc_handle *h, const char *path);
+dsmc_status dsmc_load_state_file(dsmc_handle *h, const char *path);
+
+/* --- the step: System::step (system.cpp:22-47) ------------------------- */
+dsmc_status dsmc_step(dsmc_handle *h, int nsteps);
+dsmc_status dsmc_synchronize(dsmc_handle *h);
+dsmc_status dsmc_get_counters(dsmc_handle *h, dsmc_counters *c);
+/* The reference chooses the collider from surface_interaction_model
+ * (system.cpp:343-353). */
+dsmc_status dsmc_set_wall_model(dsmc_handle *h, int model);
+
+/* --- sampling (per-cell analogue of StatisticsSampler, statisticssampler.cpp:42-65) */
+/* Accumulate per-cell moments every `every` steps (0 disables). */
+dsmc_status dsmc_set_sampling(dsmc_handle *h, int every);
+/* 7 doubles per cell of all members, global cell order:
+ * n, Sum vx, Sum vy, Sum vz, Sum vx^2, Sum vy^2, Sum vz^2; *nsamples out. */
+dsmc_status dsmc_get_cell_moments(dsmc_handle *h, double *mom7, int64_t *nsamples);
+dsmc_status dsmc_reset_cell_moments(dsmc_handle *h);
+/* FP64 totals over all particles: momentum (3) and sum |v|^2 (1). */
+dsmc_status dsmc_get_totals(dsmc_handle *h, double *tot4);
+
+/* --- cells: Cell::volume/collision_coefficient/vr_max/collision_rest ---- */
+dsmc_status dsmc_num_cells(dsmc_handle *h, int64_t *ncells);
+/* Per cell (global order): coeff, vr_max, rest (doubles), count (int32). */
+dsmc_status dsmc_get_cells(dsmc_handle *h, double *coeff, double *vr_max, double *rest, int32_t *count);
+dsmc_status dsmc_set_cells(dsmc_handle *h, const double *vr_max, const double *rest);
+/* Pairs selected and accepted per cell in the last collide phase. */
+dsmc_status dsmc_get_collide_stats(dsmc_handle *h, int32_t *pairs, int32_t *accepted);
+
+/* --- inspection for parity tests (no reference counterpart) ------------- */
+enum {
+    DSMC_OPT_PHASES = 1,        /* bit0 move, bit1 sort, bit2 collide (default 7) */
+    DSMC_OPT_RECORD = 2,        /* 1: record permutation and wall hits of each step */
+    DSMC_OPT_TOTALS = 3,        /* 1: FP64 momentum/energy totals before/after collide */
+    DSMC_OPT_LDS_CAPACITY = 4,  /* particles per cell held in LDS (0: automatic) */
+    DSMC_OPT_USE_GRAPH = 5      /* 1: replay steps from a captured HIP graph */
+};
+dsmc_status dsmc_set_option(dsmc_handle *h, int option, int64_t value);
+/* Last step: perm[new index] = index before the step; per particle (index
+ * before the step) number of wall hits and last hit voxel (linear id). */
+dsmc_status dsmc_get_last_perm(dsmc_handle *h, int32_t *perm);
+dsmc_status dsmc_get_last_wall_hits(dsmc_handle *h, int32_t *nhits, int32_t *last_voxel);
+/* Totals before (4) and after (4) the last collide phase. */
+dsmc_status dsmc_get_collide_totals(dsmc_handle *h, double *tot8);
+/* Average device time per launch (ms) of each kernel over the last
+ * dsmc_step call when timing is enabled (option 6); names as in rocprof. */
+enum { DSMC_OPT_TIMING = 6 };
+dsmc_status dsmc_get_kernel_times(dsmc_handle *h, int *nkernels, char *names, int names_len, double *ms);
+
+/* --- geometry (the step's input; complexgeometry.cpp) ------------------ */
+/* Reference packed-sphere generator (complexgeometry.cpp:401-513) and
+ * classification (928-1063) into library-owned arrays. */
+dsmc_status dsmc_geometry_packed_spheres(int n, double radius_vox, double porosity, int seed,
+                                         int nna, dsmc_world *out);
+/* Polydisperse periodic spheres, radii log-uniform in [rmin, rmax] voxels. */
+dsmc_status dsmc_geometry_poly_spheres(uint32_t nx, uint32_t ny, uint32_t nz, double rmin_vox,
+                                       double rmax_vox, double porosity, uint64_t seed, int nna,
+                                       dsmc_world *out);
+/* Write a one-rank halo world file (complexgeometry.cpp:220-318 layout). */
+dsmc_status dsmc_world_write(const char *path, const dsmc_world *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,82 @@
+"""Builds the native pieces in-tree.
+
+* master_amd/libdsmc.so   the product: HIP kernels for gfx950 + host System
+                          + C ABI (include/dsmc.h), compiled with hipcc
+* oracle/liboracle.so     the CPU oracle (test infrastructure), gcc
+* oracle/_ref/...         the reference harness, only where the reference
+                          sources exist (oracle/Makefile.ref)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "master_amd", "csrc")
+BUILD = os.path.join(ROOT, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("DSMC_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off: the kernels recompute moves bit-identically and the
+# oracle comparison assumes unfused IEEE operations.  Packed FP32 codegen is
+# disabled on this target.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
+
+LIB_SOURCES = ["dsmc_kernels.hip", "dsmc_host.cpp", "dsmc_io.cpp"]
+
+
+def _run(cmd, cwd=ROOT):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and l.strip() != "")
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError(f"build command failed ({r.returncode}): {' '.join(cmd)}")
+    return out
+
+
+def build_lib():
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in LIB_SOURCES:
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        # host translation units include the HIP runtime headers, so every
+        # source goes through hipcc as HIP
+        cmd = [HIPCC, "-x", "hip"] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        _run(cmd)
+        objs.append(obj)
+    out = os.path.join(ROOT, "master_amd", "libdsmc.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle():
+    src = os.path.join(ROOT, "oracle", "dsmc_oracle.c")
+    out = os.path.join(ROOT, "oracle", "liboracle.so")
+    _run(["gcc", "-O2", "-std=c99", "-D_DEFAULT_SOURCE", "-ffp-contract=off", "-fPIC", "-shared",
+          "-Wall", "-Wextra", "-Wno-unused-parameter", src, "-o", out + ".tmp", "-lm"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_ref(ref_root="/root/reference"):
+    if not os.path.isdir(os.path.join(ref_root, "code", "base_code", "program", "dsmc")):
+        return None
+    if shutil.which("make") is None:
+        return None
+    _run(["make", "-s", "-f", "oracle/Makefile.ref", f"REF={ref_root}", "-j8"])
+    return os.path.join(ROOT, "oracle", "_ref", "libdsmcref.so")
+
+
+def build_all():
+    return {"lib": build_lib(), "oracle": build_oracle(), "ref": build_ref()}
+
+
+if __name__ == "__main__":
+    print(build_all())

@@ -1,0 +1,331 @@
+// dsmc_device.h -- device-side building blocks of the DSMC step for gfx950.
+//
+// Particle state is structure-of-arrays, 28 bytes per particle:
+//   ox, oy, oz  float   offset inside the voxel, in [0, 1)
+//   vx, vy, vz  float   velocity, reduced units (um per 1.00294 ns)
+//   vox         uint32  voxel coordinates ix<<20 | iy<<10 | iz (member-local)
+// so the position of a particle is ((ix + ox) h_x, (iy + oy) h_y, (iz + oz) h_z)
+// and its collision cell is an exact integer function of vox.
+//
+// All arithmetic is written to be evaluated exactly as spelled (the library
+// is compiled with -ffp-contract=off): recomputing a particle's move with the
+// same inputs gives bit-identical results in any kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dsmc {
+
+enum : uint32_t { V_EMPTY = 0, V_WALL = 1, V_BOUNDARY = 2 };
+enum : uint32_t { TAG_INIT_POS = 1, TAG_INIT_VEL = 2, TAG_WALL = 3, TAG_RESEED = 4,
+                  TAG_SELECT = 5, TAG_SCATTER = 6 };
+enum : int { WALL_THERMAL = 0, WALL_SPECULAR = 1 };
+
+// Per ensemble member constants.
+struct Member {
+    int n[3];            // voxels per axis
+    int cpa[3];          // cells per axis
+    int b[3];            // voxels per cell per axis
+    uint32_t cell0;      // first global cell of the member
+    uint32_t ncell;      // cells of the member
+    uint32_t nvox;       // voxels of the member
+    uint64_t vox0;       // first voxel of the member in the voxel arrays
+    float inv_h[3];      // voxels per unit length
+    float dt;            // time step
+    float gdt;           // gravity * dt
+    int flow_dir;        // -1: no body force
+    float s_wall;        // sqrt(T_wall / m)
+    float s_gas;         // sqrt(T / m)
+    int wall_model;
+    uint32_t key_lo, key_hi;
+};
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                        uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+// (2m+1) 2^-24 with m = x >> 9: exact in float and double, in (0, 1).
+__device__ __forceinline__ float u01f(uint32_t x) {
+    return (float)((x >> 9) * 2u + 1u) * (1.0f / 16777216.0f);
+}
+__device__ __forceinline__ double u01d(uint32_t x) {
+    return (double)((x >> 9) * 2u + 1u) * (1.0 / 16777216.0);
+}
+
+__device__ __forceinline__ uint32_t pack_vox(int ix, int iy, int iz) {
+    return ((uint32_t)ix << 20) | ((uint32_t)iy << 10) | (uint32_t)iz;
+}
+__device__ __forceinline__ void unpack_vox(uint32_t v, int &ix, int &iy, int &iz) {
+    ix = (int)(v >> 20);
+    iy = (int)((v >> 10) & 1023u);
+    iz = (int)(v & 1023u);
+}
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+    i = i < 0 ? i + n : i;
+    return i >= n ? i - n : i;
+}
+
+// Global cell id of a (wrapped) voxel, cell_index_from_ijk ordering
+// (system.h:130-132): cx*cy*cz + cy*cz + cz.
+__device__ __forceinline__ uint32_t cell_of(const Member &M, int ix, int iy, int iz) {
+    int cx = ix / M.b[0], cy = iy / M.b[1], cz = iz / M.b[2];
+    return M.cell0 + (uint32_t)((cx * M.cpa[1] + cy) * M.cpa[2] + cz);
+}
+
+// Voxel-type lookup: an LDS tile covering one cell and a one-voxel halo,
+// falling back to the member's voxel array (periodic) outside it.
+struct VoxelView {
+    const uint8_t *types;     // member's types (z-fastest)
+    const uint8_t *tile;      // LDS tile
+    int tx0, ty0, tz0;        // tile origin (unwrapped voxel coords)
+    int TX, TY, TZ;           // tile extents
+    int n0, n1, n2;
+    __device__ __forceinline__ uint32_t at(int ix, int iy, int iz) const {
+        int a = ix - tx0, b = iy - ty0, c = iz - tz0;
+        if ((unsigned)a < (unsigned)TX && (unsigned)b < (unsigned)TY && (unsigned)c < (unsigned)TZ)
+            return tile[(a * TY + b) * TZ + c];
+        int x = wrapi(ix, n0), y = wrapi(iy, n1), z = wrapi(iz, n2);
+        if ((unsigned)x >= (unsigned)n0 || (unsigned)y >= (unsigned)n1 || (unsigned)z >= (unsigned)n2)
+            return V_WALL;  // lost particle: treated as solid, the mover re-seeds it
+        return types[((size_t)x * n1 + y) * n2 + z];
+    }
+};
+
+struct MoveCounters {
+    uint32_t wall_hits;
+    uint32_t reseeds;
+    uint32_t errors;
+    int32_t periodic[3];
+};
+
+// o += w*tau with the integer part carried into the voxel coordinate.
+__device__ __forceinline__ void advance1(int &i, float &o, float w, float tau) {
+    float x = o + w * tau;
+    float f = floorf(x);
+    i += (int)f;
+    x = x - f;
+    if (x >= 1.0f) { x -= 1.0f; i += 1; }
+    o = x;
+}
+
+// grid.h:82-137 in voxel units: time for the ray d + w t (d relative to the
+// boundary voxel's lower corner) to reach a face of the unit cube, checking
+// the six faces in the reference's order (x-, x+, y-, y+, z-, z+), each hit
+// point inside its closed face square, minimum positive time wins.
+__device__ __forceinline__ float face_time(float dx, float dy, float dz, float wx, float wy,
+                                           float wz, bool &no_face) {
+    float best = 1e9f;
+    float t;
+#define DSMC_FACE(T, A, WA, B, WB)                                                         \
+    t = (T);                                                                               \
+    {                                                                                      \
+        float pa = (A) + (WA) * t, pb = (B) + (WB) * t;                                    \
+        bool in = pa >= 0.0f && pa <= 1.0f && pb >= 0.0f && pb <= 1.0f;                    \
+        if (t > 0.0f && t < best && in && !isnan(t)) best = t;                             \
+    }
+    DSMC_FACE((0.0f - dx) / wx, dy, wy, dz, wz)
+    DSMC_FACE((1.0f - dx) / wx, dy, wy, dz, wz)
+    DSMC_FACE((0.0f - dy) / wy, dx, wx, dz, wz)
+    DSMC_FACE((1.0f - dy) / wy, dx, wx, dz, wz)
+    DSMC_FACE((0.0f - dz) / wz, dx, wx, dy, wy)
+    DSMC_FACE((1.0f - dz) / wz, dx, wx, dy, wy)
+#undef DSMC_FACE
+    no_face = (best == 1e9f);
+    if (best < 1e-5f) return 0.0f;
+    return best - 1e-7f;
+}
+
+// Re-seed uniformly in the pore space (System::find_position,
+// system.cpp:384-396): a uniform integer voxel pick rejected on solid voxels,
+// then a uniform offset; the velocity is kept.
+__device__ __forceinline__ bool reseed(const Member &M, const uint8_t *types, uint32_t step,
+                                       uint32_t pidx, int &ix, int &iy, int &iz, float &ox,
+                                       float &oy, float &oz) {
+    for (uint32_t attempt = 0; attempt < 100000u; attempt++) {
+        uint4 o = philox(TAG_RESEED, step, pidx, attempt, M.key_lo, M.key_hi);
+        uint32_t lin = __umulhi(o.x, M.nvox);
+        if (types[lin] != V_EMPTY) continue;
+        uint32_t nyz = (uint32_t)M.n[1] * (uint32_t)M.n[2];
+        ix = (int)(lin / nyz);
+        iy = (int)((lin / (uint32_t)M.n[2]) % (uint32_t)M.n[1]);
+        iz = (int)(lin % (uint32_t)M.n[2]);
+        ox = u01f(o.y);
+        oy = u01f(o.z);
+        oz = u01f(o.w);
+        return true;
+    }
+    return false;
+}
+
+// Surface interaction (colliderthermal.cpp:13-46, colliderspecular.cpp:8-32).
+// frame = 9 floats: normal, tangent1, tangent2.
+__device__ __forceinline__ void wall_collide(const Member &M, const float *frame, uint32_t step,
+                                             uint32_t pidx, uint32_t hit, float &vx, float &vy,
+                                             float &vz) {
+    float nx = frame[0], ny = frame[1], nz = frame[2];
+    float ax = frame[3], ay = frame[4], az = frame[5];
+    float bx = frame[6], by = frame[7], bz = frame[8];
+    float vn, vt1, vt2;
+    if (M.wall_model == WALL_SPECULAR) {
+        float un = nx * vx + ny * vy + nz * vz;
+        float u1 = ax * vx + ay * vy + az * vz;
+        float u2 = bx * vx + by * vy + bz * vz;
+        vn = -un; vt1 = u1; vt2 = u2;
+    } else {
+        uint4 o = philox(TAG_WALL, step, pidx, hit, M.key_lo, M.key_hi);
+        float rr = sqrtf(-2.0f * logf(u01f(o.y)));
+        float sp, cp;
+        sincospif(2.0f * u01f(o.z), &sp, &cp);
+        vn = M.s_wall * sqrtf(-2.0f * logf(u01f(o.x)));
+        vt1 = M.s_wall * (rr * cp);
+        vt2 = M.s_wall * (rr * sp);
+    }
+    vx = vn * nx + vt1 * ax + vt2 * bx;
+    vy = vn * ny + vt1 * ay + vt2 * by;
+    vz = vn * nz + vt1 * az + vt2 * bz;
+}
+
+// One particle's step of System::accelerate (system.cpp:133-144) and
+// MoleculeMover::move_molecule (moleculemover.cpp:50-126, the recursion as a
+// loop) followed by apply_periodic_boundary_conditions (moleculemover.h:29-38).
+// Coordinates stay unwrapped during the flight (the reference's halo grid)
+// and are wrapped once at the end.  Returns the last wall-hit voxel's linear
+// id or -1.
+__device__ __forceinline__ int32_t move_particle(const Member &M, const VoxelView &V,
+                                                 const float *frames, uint32_t step,
+                                                 uint32_t pidx, int &ix, int &iy, int &iz,
+                                                 float &ox, float &oy, float &oz, float &vx,
+                                                 float &vy, float &vz, MoveCounters &cnt,
+                                                 uint32_t &nhit) {
+    if (M.flow_dir == 0) vx = vx + M.gdt;
+    else if (M.flow_dir == 1) vy = vy + M.gdt;
+    else if (M.flow_dir == 2) vz = vz + M.gdt;
+    float dt = M.dt;
+    int32_t last = -1;
+    uint32_t hit = 0;
+    bool reseeded = false;
+    for (int depth = 0;; depth++) {
+        float tau = dt;
+        if (V.at(ix, iy, iz) != V_EMPTY) { reseeded = true; break; }
+        float wx = vx * M.inv_h[0], wy = vy * M.inv_h[1], wz = vz * M.inv_h[2];
+        advance1(ix, ox, wx, tau);
+        advance1(iy, oy, wy, tau);
+        advance1(iz, oz, wz, tau);
+        uint32_t t = V.at(ix, iy, iz);
+        if (t >= V_WALL) {
+            int count = 0;
+            bool resumed = false, bad = false;
+            while (t != V_BOUNDARY) {
+                if (++count > 100) { bad = true; break; }
+                if (t == V_WALL) {
+                    tau = tau / 2.0f;
+                    advance1(ix, ox, wx, -tau);
+                    advance1(iy, oy, wy, -tau);
+                    advance1(iz, oz, wz, -tau);
+                    t = V.at(ix, iy, iz);
+                } else {
+                    dt = dt - tau;
+                    resumed = true;
+                    break;
+                }
+            }
+            if (bad) { reseeded = true; break; }
+            if (resumed) {
+                if (dt > 1e-5f && depth < 10) continue;
+                break;
+            }
+            int bx = ix, by = iy, bz = iz;
+            count = 0;
+            while (t == V_BOUNDARY) {
+                if (++count > 100) { bad = true; break; }
+                bx = ix; by = iy; bz = iz;
+                advance1(ix, ox, wx, -tau);
+                advance1(iy, oy, wy, -tau);
+                advance1(iz, oz, wz, -tau);
+                bool no_face;
+                tau = face_time((float)(ix - bx) + ox, (float)(iy - by) + oy, (float)(iz - bz) + oz,
+                                wx, wy, wz, no_face);
+                if (no_face) { cnt.errors++; bad = true; break; }
+                advance1(ix, ox, wx, tau);
+                advance1(iy, oy, wy, tau);
+                advance1(iz, oz, wz, tau);
+                t = V.at(ix, iy, iz);
+            }
+            if (bad) { reseeded = true; break; }
+            dt = dt - tau;
+            int gx = wrapi(bx, M.n[0]), gy = wrapi(by, M.n[1]), gz = wrapi(bz, M.n[2]);
+            size_t lin = ((size_t)gx * M.n[1] + gy) * M.n[2] + gz;
+            wall_collide(M, frames + 9 * lin, step, pidx, hit, vx, vy, vz);
+            hit++;
+            cnt.wall_hits++;
+            last = (int32_t)lin;
+        } else {
+            dt = 0.0f;
+        }
+        if (!(dt > 1e-5f && depth < 10)) break;
+    }
+    if (reseeded) {
+        cnt.reseeds++;
+        if (!reseed(M, V.types, step, pidx, ix, iy, iz, ox, oy, oz)) {
+            // no pore voxel found: keep the particle inside the box
+            cnt.errors++;
+            ix = ((ix % M.n[0]) + M.n[0]) % M.n[0];
+            iy = ((iy % M.n[1]) + M.n[1]) % M.n[1];
+            iz = ((iz % M.n[2]) + M.n[2]) % M.n[2];
+        }
+    } else {
+        if (ix >= M.n[0]) { ix -= M.n[0]; cnt.periodic[0]++; } else if (ix < 0) { ix += M.n[0]; cnt.periodic[0]--; }
+        if (iy >= M.n[1]) { iy -= M.n[1]; cnt.periodic[1]++; } else if (iy < 0) { iy += M.n[1]; cnt.periodic[1]--; }
+        if (iz >= M.n[2]) { iz -= M.n[2]; cnt.periodic[2]++; } else if (iz < 0) { iz += M.n[2]; cnt.periodic[2]--; }
+        if ((unsigned)ix >= (unsigned)M.n[0] || (unsigned)iy >= (unsigned)M.n[1] ||
+            (unsigned)iz >= (unsigned)M.n[2]) {
+            // more than one box length in one step: cannot happen for a valid dt
+            cnt.errors++;
+            cnt.reseeds++;
+            if (!reseed(M, V.types, step, pidx, ix, iy, iz, ox, oy, oz)) {
+                ix = ((ix % M.n[0]) + M.n[0]) % M.n[0];
+                iy = ((iy % M.n[1]) + M.n[1]) % M.n[1];
+                iz = ((iz % M.n[2]) + M.n[2]) % M.n[2];
+            }
+        }
+    }
+    nhit = hit;
+    return last;
+}
+
+// Wave-level helpers (wave64).
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double y = __shfl_xor(x, off, 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+}  // namespace dsmc

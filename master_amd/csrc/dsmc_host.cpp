@@ -1,0 +1,1028 @@
+// dsmc_host.cpp -- the host-side System behind the C ABI (include/dsmc.h).
+//
+// Mirrors the reference System (system.h:29-120): initialize() builds the
+// cells (system.cpp:432-490) and seeds particles (398-430); step() runs
+// accelerate, move, re-bin and collide (22-47).  All particle state lives in
+// HBM in two structure-of-arrays buffers that alternate every step; the host
+// only enqueues kernels on the handle's stream.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+#include "../../include/dsmc.h"
+#include "dsmc_internal.h"
+#include "dsmc_kernels.h"
+
+using namespace dsmc;
+
+namespace {
+
+struct MemberHost {
+    dsmc_settings s;
+    uint32_t n[3];
+    std::vector<uint8_t> types;
+    std::vector<float> frames;      // 9 floats per voxel: normal, tangent1, tangent2
+    float porosity;
+    Member dev;
+    std::vector<double> coeff;      // per member cell
+    int64_t first = 0, count = 0;   // particle range right after seeding
+};
+
+struct Timer {
+    std::string name;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t used = 0;
+};
+
+}  // namespace
+
+struct dsmc_handle {
+    std::string err;
+    hipStream_t stream = nullptr;
+    std::vector<MemberHost> mem;
+    uint32_t ncells = 0;
+    uint64_t nvox = 0;
+    // device: geometry and cells
+    Member *d_members = nullptr;
+    uint32_t *d_cell_member = nullptr;
+    uint8_t *d_types = nullptr;
+    float *d_frames = nullptr;
+    double *d_coeff = nullptr, *d_vrmax = nullptr, *d_rest = nullptr, *d_mom = nullptr;
+    int32_t *d_pairs = nullptr, *d_acc = nullptr;
+    uint32_t *d_start[2] = {nullptr, nullptr};
+    int scur = 0;
+    uint32_t *d_leave = nullptr, *d_arrive = nullptr, *d_bstart = nullptr, *d_bfill = nullptr;
+    unsigned long long *d_bsums = nullptr;
+    uint32_t nblocks = 0;
+    uint32_t *d_step = nullptr, *d_mvcount = nullptr;
+    unsigned long long *d_cnt = nullptr;
+    double *d_partial = nullptr;
+    // device: particles
+    int64_t N = 0;
+    float *ox[2] = {}, *oy[2] = {}, *oz[2] = {}, *vx[2] = {}, *vy[2] = {}, *vz[2] = {};
+    uint32_t *vox[2] = {};
+    int cur = 0;
+    uint32_t *d_mv_idx = nullptr, *d_mv_cell = nullptr, *d_bucket = nullptr;
+    int32_t *d_rec_perm = nullptr, *d_rec_nhit = nullptr, *d_rec_last = nullptr;
+    // options
+    int phases = 7;
+    bool record = false, totals = false, timing = false, use_graph = false;
+    int sample_every = 0;
+    uint32_t lds_cap = 0;
+    bool lds_auto = true;
+    // host mirrors of System counters
+    int64_t steps = 0;
+    double t = 0;
+    int64_t nsamples = 0;
+    bool initialized = false;
+    int64_t max_occ = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraph_t graph_src[2] = {nullptr, nullptr};
+    bool graph_sample[2] = {false, false};
+    std::vector<Timer> timers;
+};
+
+namespace {
+
+const int TOT_BLOCKS = 512;
+
+#define HIPCHK(h, call)                                                             \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);           \
+            return DSMC_ERR_HIP;                                                    \
+        }                                                                           \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **)p, n * sizeof(T));
+}
+
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+dsmc_status fail(dsmc_handle *h, dsmc_status st, const std::string &msg) {
+    if (h) h->err = msg;
+    return st;
+}
+
+// System::validate_number_of_cells (system.cpp:221-239), one rank.
+bool cells_valid(const dsmc_settings &s, const uint32_t n[3]) {
+    int c[3] = {s.cells_x, s.cells_y, s.cells_z};
+    for (int a = 0; a < 3; a++)
+        if (c[a] <= 0 || n[a] % (uint32_t)c[a] != 0) return false;
+    return true;
+}
+
+dsmc_status build_member(dsmc_handle *h, const dsmc_settings &s, const dsmc_world &w, MemberHost &m) {
+    if (!w.types || !w.normals || !w.tangents1 || !w.tangents2)
+        return fail(h, DSMC_ERR_INVALID_ARGUMENT, "world arrays missing");
+    if (w.nx < 2 || w.ny < 2 || w.nz < 2 || w.nx > 1024 || w.ny > 1024 || w.nz > 1024)
+        return fail(h, DSMC_ERR_INVALID_ARGUMENT, "world dimensions must be in [2, 1024]");
+    m.s = s;
+    m.n[0] = w.nx; m.n[1] = w.ny; m.n[2] = w.nz;
+    if (!cells_valid(s, m.n)) return fail(h, DSMC_ERR_CONFIG, "voxel-cell configuration not valid (system.cpp:221-239)");
+    int b[3] = {(int)w.nx / s.cells_x, (int)w.ny / s.cells_y, (int)w.nz / s.cells_z};
+    if ((b[0] + 2) * (b[1] + 2) * (b[2] + 2) > MAX_TILE)
+        return fail(h, DSMC_ERR_CONFIG, "collision cells larger than 8 voxels per axis are not supported");
+    if (!(s.dt > 0) || !(s.mass > 0) || !(s.diam > 0) || s.atoms_per_molecule <= 0 || !(s.Lx > 0) ||
+        !(s.Ly > 0) || !(s.Lz > 0))
+        return fail(h, DSMC_ERR_INVALID_ARGUMENT, "dt, mass, diam, atoms_per_molecule and L must be positive");
+    size_t nv = (size_t)w.nx * w.ny * w.nz;
+    m.types.assign(w.types, w.types + nv);
+    for (size_t i = 0; i < nv; i++)
+        if (m.types[i] > 2) return fail(h, DSMC_ERR_INVALID_ARGUMENT, "voxel type out of range");
+    m.frames.resize(9 * nv);
+    for (size_t i = 0; i < nv; i++)
+        for (int a = 0; a < 3; a++) {
+            m.frames[9 * i + a] = w.normals[3 * i + a];
+            m.frames[9 * i + 3 + a] = w.tangents1[3 * i + a];
+            m.frames[9 * i + 6 + a] = w.tangents2[3 * i + a];
+        }
+    m.porosity = w.porosity;
+    Member &d = m.dev;
+    std::memset(&d, 0, sizeof d);
+    double L[3] = {s.Lx, s.Ly, s.Lz};
+    int cells[3] = {s.cells_x, s.cells_y, s.cells_z};
+    for (int a = 0; a < 3; a++) {
+        d.n[a] = (int)m.n[a];
+        d.cpa[a] = cells[a];
+        d.b[a] = b[a];
+        // grid.cpp:44-46: one_over_length * global_n stored as float
+        d.inv_h[a] = (float)((1.0 / L[a]) * m.n[a]);
+    }
+    d.ncell = (uint32_t)(cells[0] * cells[1] * cells[2]);
+    d.nvox = (uint32_t)nv;
+    d.dt = (float)s.dt;
+    // System::accelerate skips when flow_direction < 0 or gravity == 0 (133-134)
+    d.flow_dir = (s.flow_direction < 0 || s.flow_direction > 2 || s.gravity == 0) ? -1 : s.flow_direction;
+    d.gdt = (float)(s.gravity * s.dt);
+    d.s_wall = (float)std::sqrt(s.wall_temperature / s.mass);
+    d.s_gas = (float)std::sqrt(s.temperature / s.mass);
+    d.wall_model = s.wall_model;
+    uint64_t key = (uint64_t)(int64_t)s.seed;
+    d.key_lo = (uint32_t)key;
+    d.key_hi = (uint32_t)(key >> 32);
+    // setup_cells (system.cpp:432-490) + Cell::update_volume (cell.cpp:27-40)
+    m.coeff.assign(d.ncell, 0.0);
+    std::vector<int> pixels(d.ncell, 0), total(d.ncell, 0);
+    for (uint32_t i = 0; i < m.n[0]; i++)
+        for (uint32_t j = 0; j < m.n[1]; j++)
+            for (uint32_t k = 0; k < m.n[2]; k++) {
+                uint32_t c = ((i / b[0]) * cells[1] + j / b[1]) * cells[2] + k / b[2];
+                total[c]++;
+                pixels[c] += m.types[((size_t)i * m.n[1] + j) * m.n[2] + k] < 1;
+            }
+    double sysvol = s.Lx * s.Ly * s.Lz;
+    int nc = (int)d.ncell;
+    for (uint32_t c = 0; c < d.ncell; c++) {
+        if (pixels[c] == 0) continue;
+        double vol = sysvol / (nc) * (float)pixels[c] / total[c];
+        m.coeff[c] = 0.5 * (double)s.atoms_per_molecule * M_PI * s.diam * s.diam * s.dt / vol;
+    }
+    return DSMC_OK;
+}
+
+dsmc_status upload_geometry(dsmc_handle *h) {
+    uint32_t cells = 0;
+    uint64_t vox = 0;
+    for (auto &m : h->mem) {
+        m.dev.cell0 = cells;
+        m.dev.vox0 = vox;
+        cells += m.dev.ncell;
+        vox += m.dev.nvox;
+    }
+    h->ncells = cells;
+    h->nvox = vox;
+    std::vector<Member> members;
+    std::vector<uint32_t> cmem(cells);
+    std::vector<uint8_t> types(vox);
+    std::vector<float> frames(9 * vox);
+    std::vector<double> coeff(cells), vr(cells), rest(cells, 0.0);
+    for (size_t mi = 0; mi < h->mem.size(); mi++) {
+        auto &m = h->mem[mi];
+        members.push_back(m.dev);
+        std::copy(m.types.begin(), m.types.end(), types.begin() + m.dev.vox0);
+        std::copy(m.frames.begin(), m.frames.end(), frames.begin() + 9 * m.dev.vox0);
+        double vr0 = 3 * std::sqrt(m.s.temperature);   // system.cpp:262, 477: mass omitted
+        for (uint32_t c = 0; c < m.dev.ncell; c++) {
+            cmem[m.dev.cell0 + c] = (uint32_t)mi;
+            coeff[m.dev.cell0 + c] = m.coeff[c];
+            vr[m.dev.cell0 + c] = vr0;
+        }
+    }
+    h->nblocks = (cells + SCAN_TILE - 1) / SCAN_TILE;
+    if (h->nblocks > SCAN_THREADS * 64) return fail(h, DSMC_ERR_CONFIG, "too many cells");
+    HIPCHK(h, hipStreamCreate(&h->stream));
+    HIPCHK(h, dalloc(&h->d_members, members.size()));
+    HIPCHK(h, dalloc(&h->d_cell_member, cells));
+    HIPCHK(h, dalloc(&h->d_types, vox));
+    HIPCHK(h, dalloc(&h->d_frames, 9 * vox));
+    HIPCHK(h, dalloc(&h->d_coeff, cells));
+    HIPCHK(h, dalloc(&h->d_vrmax, cells));
+    HIPCHK(h, dalloc(&h->d_rest, cells));
+    HIPCHK(h, dalloc(&h->d_mom, 7 * (size_t)cells));
+    HIPCHK(h, dalloc(&h->d_pairs, cells));
+    HIPCHK(h, dalloc(&h->d_acc, cells));
+    HIPCHK(h, dalloc(&h->d_start[0], cells + 1));
+    HIPCHK(h, dalloc(&h->d_start[1], cells + 1));
+    HIPCHK(h, dalloc(&h->d_leave, cells));
+    HIPCHK(h, dalloc(&h->d_arrive, cells));
+    HIPCHK(h, dalloc(&h->d_bstart, cells + 1));
+    HIPCHK(h, dalloc(&h->d_bfill, cells));
+    HIPCHK(h, dalloc(&h->d_bsums, h->nblocks));
+    HIPCHK(h, dalloc(&h->d_step, 1));
+    HIPCHK(h, dalloc(&h->d_mvcount, 1));
+    HIPCHK(h, dalloc(&h->d_cnt, CNT_NUM));
+    HIPCHK(h, dalloc(&h->d_partial, 2 * TOT_BLOCKS * 4));
+    HIPCHK(h, hipMemcpy(h->d_members, members.data(), members.size() * sizeof(Member), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_cell_member, cmem.data(), cells * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_types, types.data(), vox, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_frames, frames.data(), 9 * vox * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_coeff, coeff.data(), cells * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_vrmax, vr.data(), cells * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_rest, rest.data(), cells * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemset(h->d_mom, 0, 7 * (size_t)cells * sizeof(double)));
+    HIPCHK(h, hipMemset(h->d_pairs, 0, cells * sizeof(int32_t)));
+    HIPCHK(h, hipMemset(h->d_acc, 0, cells * sizeof(int32_t)));
+    HIPCHK(h, hipMemset(h->d_start[0], 0, (cells + 1) * sizeof(uint32_t)));
+    HIPCHK(h, hipMemset(h->d_start[1], 0, (cells + 1) * sizeof(uint32_t)));
+    HIPCHK(h, hipMemset(h->d_step, 0, sizeof(uint32_t)));
+    HIPCHK(h, hipMemset(h->d_cnt, 0, CNT_NUM * sizeof(unsigned long long)));
+    return DSMC_OK;
+}
+
+void free_particles(dsmc_handle *h) {
+    for (int b = 0; b < 2; b++) {
+        dfree(h->ox[b]); dfree(h->oy[b]); dfree(h->oz[b]);
+        dfree(h->vx[b]); dfree(h->vy[b]); dfree(h->vz[b]);
+        dfree(h->vox[b]);
+    }
+    dfree(h->d_mv_idx); dfree(h->d_mv_cell); dfree(h->d_bucket);
+    dfree(h->d_rec_perm); dfree(h->d_rec_nhit); dfree(h->d_rec_last);
+}
+
+void drop_graphs(dsmc_handle *h) {
+    for (int i = 0; i < 2; i++) {
+        if (h->graph[i]) (void)hipGraphExecDestroy(h->graph[i]);
+        if (h->graph_src[i]) (void)hipGraphDestroy(h->graph_src[i]);
+        h->graph[i] = nullptr;
+        h->graph_src[i] = nullptr;
+    }
+}
+
+dsmc_status alloc_particles(dsmc_handle *h, int64_t n) {
+    free_particles(h);
+    drop_graphs(h);
+    if (n > (int64_t)0xFFFFFFF0ll) return fail(h, DSMC_ERR_CAPACITY, "more than 2^32 particles");
+    h->N = n;
+    for (int b = 0; b < 2; b++) {
+        HIPCHK(h, dalloc(&h->ox[b], n)); HIPCHK(h, dalloc(&h->oy[b], n)); HIPCHK(h, dalloc(&h->oz[b], n));
+        HIPCHK(h, dalloc(&h->vx[b], n)); HIPCHK(h, dalloc(&h->vy[b], n)); HIPCHK(h, dalloc(&h->vz[b], n));
+        HIPCHK(h, dalloc(&h->vox[b], n));
+    }
+    HIPCHK(h, dalloc(&h->d_mv_idx, n));
+    HIPCHK(h, dalloc(&h->d_mv_cell, n));
+    HIPCHK(h, dalloc(&h->d_bucket, n));
+    if (h->record) {
+        HIPCHK(h, dalloc(&h->d_rec_perm, n));
+        HIPCHK(h, dalloc(&h->d_rec_nhit, n));
+        HIPCHK(h, dalloc(&h->d_rec_last, n));
+    }
+    h->cur = 0;
+    return DSMC_OK;
+}
+
+// ---------------------------------------------------------------- timing
+struct Scope {
+    dsmc_handle *h;
+    Timer *t = nullptr;
+    hipEvent_t e1 = nullptr;
+    Scope(dsmc_handle *h_, const char *name) : h(h_) {
+        if (!h->timing) return;
+        for (auto &x : h->timers)
+            if (x.name == name) t = &x;
+        if (!t) {
+            h->timers.push_back(Timer{name, {}, 0});
+            t = &h->timers.back();
+        }
+        if (t->used == t->ev.size()) {
+            hipEvent_t a, b;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            t->ev.push_back({a, b});
+        }
+        (void)hipEventRecord(t->ev[t->used].first, h->stream);
+        e1 = t->ev[t->used].second;
+        t->used++;
+    }
+    ~Scope() {
+        if (e1) (void)hipEventRecord(e1, h->stream);
+    }
+};
+
+uint32_t auto_lds_cap(int64_t max_occ) {
+    int64_t k = (int64_t)(max_occ * 1.25) + 64;
+    k = (k + 63) / 64 * 64;
+    if (k < 128) k = 128;
+    if (k > 4096) k = 4096;
+    return (uint32_t)k;
+}
+
+dsmc_status refresh_occupancy(dsmc_handle *h) {
+    std::vector<uint32_t> st(h->ncells + 1);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(st.data(), h->d_start[h->scur], (h->ncells + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    int64_t mx = 0;
+    for (uint32_t c = 0; c < h->ncells; c++) mx = std::max<int64_t>(mx, (int64_t)st[c + 1] - st[c]);
+    h->max_occ = mx;
+    if (h->lds_auto) {
+        uint32_t k = auto_lds_cap(mx);
+        if (k != h->lds_cap) drop_graphs(h);
+        h->lds_cap = k;
+    }
+    return DSMC_OK;
+}
+
+GatherArgs gather_args(dsmc_handle *h, bool collide, bool sample) {
+    GatherArgs g;
+    std::memset(&g, 0, sizeof g);
+    int a = h->cur, b = 1 - h->cur;
+    g.ox = h->ox[a]; g.oy = h->oy[a]; g.oz = h->oz[a]; g.vx = h->vx[a]; g.vy = h->vy[a]; g.vz = h->vz[a];
+    g.vox = h->vox[a];
+    g.ox2 = h->ox[b]; g.oy2 = h->oy[b]; g.oz2 = h->oz[b]; g.vx2 = h->vx[b]; g.vy2 = h->vy[b]; g.vz2 = h->vz[b];
+    g.vox2 = h->vox[b];
+    g.start = h->d_start[h->scur];
+    g.start_new = h->d_start[1 - h->scur];
+    g.bstart = h->d_bstart;
+    g.bucket = h->d_bucket;
+    g.members = h->d_members;
+    g.cell_member = h->d_cell_member;
+    g.coeff = h->d_coeff;
+    g.vr_max = h->d_vrmax;
+    g.rest = h->d_rest;
+    g.stat_pairs = h->d_pairs;
+    g.stat_acc = h->d_acc;
+    g.moments = h->d_mom;
+    g.step = h->d_step;
+    g.counters = h->d_cnt;
+    g.rec_perm = h->record ? h->d_rec_perm : nullptr;
+    g.ncells = h->ncells;
+    g.cell_begin = 0;
+    g.lds_cap = h->lds_cap;
+    g.do_collide = collide ? 1 : 0;
+    g.do_sample = sample ? 1 : 0;
+    return g;
+}
+
+ScanArgs scan_args(dsmc_handle *h) {
+    ScanArgs s;
+    s.start = h->d_start[h->scur];
+    s.leave = h->d_leave;
+    s.arrive = h->d_arrive;
+    s.start_new = h->d_start[1 - h->scur];
+    s.bstart = h->d_bstart;
+    s.bfill = h->d_bfill;
+    s.block_sums = h->d_bsums;
+    s.ncells = h->ncells;
+    s.n_blocks = h->nblocks;
+    return s;
+}
+
+BucketArgs bucket_args(dsmc_handle *h) {
+    BucketArgs b;
+    b.mv_idx = h->d_mv_idx;
+    b.mv_cell = h->d_mv_cell;
+    b.mv_count = h->d_mvcount;
+    b.bstart = h->d_bstart;
+    b.bfill = h->d_bfill;
+    b.bucket = h->d_bucket;
+    return b;
+}
+
+// Sort a freshly written, unsorted buffer into cell order (stable by index).
+dsmc_status sort_unsorted(dsmc_handle *h, const std::vector<uint32_t> &part_member) {
+    hipStream_t s = h->stream;
+    uint32_t *d_pm = nullptr;
+    HIPCHK(h, dalloc(&d_pm, h->N));
+    HIPCHK(h, hipMemcpyAsync(d_pm, part_member.data(), h->N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(h, launch_fill_u32(h->d_start[h->scur], 0u, h->ncells + 1, s));
+    HIPCHK(h, launch_fill_u32(h->d_leave, 0u, h->ncells, s));
+    HIPCHK(h, launch_fill_u32(h->d_arrive, 0u, h->ncells, s));
+    ClassifyArgs c;
+    c.vox = h->vox[h->cur];
+    c.part_member = d_pm;
+    c.members = h->d_members;
+    c.arrive = h->d_arrive;
+    c.mv_idx = h->d_mv_idx;
+    c.mv_cell = h->d_mv_cell;
+    c.mv_count = h->d_mvcount;
+    c.n = (uint32_t)h->N;
+    HIPCHK(h, launch_classify_all(c, s));
+    HIPCHK(h, launch_scan(scan_args(h), s));
+    HIPCHK(h, launch_bucket(bucket_args(h), s));
+    // LDS holds the largest cell; at this point the occupancy is not known
+    // yet, so the overflow path handles anything above the default.
+    uint32_t keep = h->lds_cap;
+    if (h->lds_cap == 0) h->lds_cap = 1024;
+    HIPCHK(h, launch_gather_collide(gather_args(h, false, false), h->ncells, s));
+    h->lds_cap = keep;
+    h->cur = 1 - h->cur;
+    h->scur = 1 - h->scur;
+    HIPCHK(h, hipStreamSynchronize(s));
+    dfree(d_pm);
+    return refresh_occupancy(h);
+}
+
+dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
+    hipStream_t s = h->stream;
+    HIPCHK(h, launch_begin_step(h->d_step, h->d_mvcount, s));
+    HIPCHK(h, launch_fill_u32(h->d_arrive, 0u, h->ncells, s));
+    if (h->phases & 1) {
+        MoveArgs m;
+        int a = h->cur;
+        m.ox = h->ox[a]; m.oy = h->oy[a]; m.oz = h->oz[a]; m.vx = h->vx[a]; m.vy = h->vy[a]; m.vz = h->vz[a];
+        m.vox = h->vox[a];
+        m.start = h->d_start[h->scur];
+        m.leave = h->d_leave;
+        m.arrive = h->d_arrive;
+        m.mv_idx = h->d_mv_idx;
+        m.mv_cell = h->d_mv_cell;
+        m.mv_count = h->d_mvcount;
+        m.members = h->d_members;
+        m.cell_member = h->d_cell_member;
+        m.types = h->d_types;
+        m.frames = h->d_frames;
+        m.step = h->d_step;
+        m.counters = h->d_cnt;
+        m.rec_nhit = h->record ? h->d_rec_nhit : nullptr;
+        m.rec_last = h->record ? h->d_rec_last : nullptr;
+        m.cell_begin = 0;
+        Scope sc(h, "k_move");
+        HIPCHK(h, launch_move(m, h->ncells, s));
+    } else {
+        HIPCHK(h, launch_fill_u32(h->d_leave, 0u, h->ncells, s));
+    }
+    if (h->totals) {
+        int a = h->cur;
+        HIPCHK(h, launch_totals_partial(h->vx[a], h->vy[a], h->vz[a], (uint32_t)h->N, h->d_partial, TOT_BLOCKS, s));
+    }
+    {
+        Scope sc(h, "k_scan");
+        HIPCHK(h, launch_scan(scan_args(h), s));
+    }
+    {
+        Scope sc(h, "k_bucket");
+        HIPCHK(h, launch_bucket(bucket_args(h), s));
+    }
+    {
+        Scope sc(h, "k_gather_collide");
+        HIPCHK(h, launch_gather_collide(gather_args(h, (h->phases & 4) != 0, sample), h->ncells, s));
+    }
+    if (h->totals) {
+        int b = 1 - h->cur;
+        HIPCHK(h, launch_totals_partial(h->vx[b], h->vy[b], h->vz[b], (uint32_t)h->N,
+                                        h->d_partial + TOT_BLOCKS * 4, TOT_BLOCKS, s));
+    }
+    h->cur = 1 - h->cur;
+    h->scur = 1 - h->scur;
+    return DSMC_OK;
+}
+
+dsmc_status capture_graph(dsmc_handle *h, int parity, bool sample) {
+    // Two steps, so the buffers return to the parity the graph starts from.
+    int save_cur = h->cur, save_scur = h->scur;
+    bool save_timing = h->timing;
+    h->timing = false;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    dsmc_status st = enqueue_step(h, sample);
+    if (st == DSMC_OK) st = enqueue_step(h, sample);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    h->timing = save_timing;
+    h->cur = save_cur;
+    h->scur = save_scur;
+    if (st != DSMC_OK) return st;
+    if (e != hipSuccess) return fail(h, DSMC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    HIPCHK(h, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    h->graph[parity] = ex;
+    h->graph_src[parity] = g;
+    h->graph_sample[parity] = sample;
+    return DSMC_OK;
+}
+
+dsmc_status seed_members(dsmc_handle *h, const std::vector<int64_t> &counts) {
+    int64_t total = 0;
+    for (size_t m = 0; m < h->mem.size(); m++) {
+        h->mem[m].first = total;
+        h->mem[m].count = counts[m];
+        total += counts[m];
+    }
+    dsmc_status st = alloc_particles(h, total);
+    if (st != DSMC_OK) return st;
+    std::vector<uint32_t> pm(total), first(h->mem.size());
+    for (size_t m = 0; m < h->mem.size(); m++) {
+        first[m] = (uint32_t)h->mem[m].first;
+        std::fill(pm.begin() + h->mem[m].first, pm.begin() + h->mem[m].first + h->mem[m].count, (uint32_t)m);
+    }
+    uint32_t *d_pm = nullptr, *d_first = nullptr;
+    HIPCHK(h, dalloc(&d_pm, total));
+    HIPCHK(h, dalloc(&d_first, first.size()));
+    HIPCHK(h, hipMemcpy(d_pm, pm.data(), total * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(d_first, first.data(), first.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    SeedArgs a;
+    int c = h->cur;
+    a.ox = h->ox[c]; a.oy = h->oy[c]; a.oz = h->oz[c]; a.vx = h->vx[c]; a.vy = h->vy[c]; a.vz = h->vz[c];
+    a.vox = h->vox[c];
+    a.part_member = d_pm;
+    a.member_first = d_first;
+    a.members = h->d_members;
+    a.types = h->d_types;
+    a.counters = h->d_cnt;
+    a.n = (uint32_t)total;
+    HIPCHK(h, launch_seed(a, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    dfree(d_first);
+    dfree(d_pm);
+    st = sort_unsorted(h, pm);
+    if (st != DSMC_OK) return st;
+    h->initialized = true;
+    return DSMC_OK;
+}
+
+int64_t reference_count(const MemberHost &m) {
+    // system.cpp:313-323: N = density * (L^3 * porosity) / Fn, truncated
+    double volume = m.s.Lx * m.s.Ly * m.s.Lz;
+    double volume_per_cpu = volume / 1;
+    volume = volume_per_cpu * (double)m.porosity;
+    return (int64_t)(m.s.density * volume / (double)m.s.atoms_per_molecule);
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char *dsmc_last_error(const dsmc_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+dsmc_status dsmc_create_ensemble(int n, const dsmc_settings *s, const dsmc_world *w, dsmc_handle **out) {
+    if (!out || !s || !w || n <= 0) return DSMC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    dsmc_handle *h = new dsmc_handle;
+    h->mem.resize(n);
+    for (int i = 0; i < n; i++) {
+        dsmc_status st = build_member(h, s[i], w[i], h->mem[i]);
+        if (st != DSMC_OK) {
+            std::fprintf(stderr, "dsmc_create: %s\n", h->err.c_str());
+            delete h;
+            return st;
+        }
+    }
+    dsmc_status st = upload_geometry(h);
+    if (st != DSMC_OK) {
+        std::fprintf(stderr, "dsmc_create: %s\n", h->err.c_str());
+        dsmc_destroy(h);
+        return st;
+    }
+    *out = h;
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_create(const dsmc_settings *s, const dsmc_world *w, dsmc_handle **out) {
+    return dsmc_create_ensemble(1, s, w, out);
+}
+
+void dsmc_destroy(dsmc_handle *h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graphs(h);
+    free_particles(h);
+    dfree(h->d_members); dfree(h->d_cell_member); dfree(h->d_types); dfree(h->d_frames);
+    dfree(h->d_coeff); dfree(h->d_vrmax); dfree(h->d_rest); dfree(h->d_mom);
+    dfree(h->d_pairs); dfree(h->d_acc);
+    dfree(h->d_start[0]); dfree(h->d_start[1]);
+    dfree(h->d_leave); dfree(h->d_arrive); dfree(h->d_bstart); dfree(h->d_bfill); dfree(h->d_bsums);
+    dfree(h->d_step); dfree(h->d_mvcount); dfree(h->d_cnt); dfree(h->d_partial);
+    for (auto &t : h->timers)
+        for (auto &e : t.ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+dsmc_status dsmc_initialize(dsmc_handle *h, int64_t *n_out) {
+    if (!h) return DSMC_ERR_INVALID_ARGUMENT;
+    std::vector<int64_t> counts;
+    int64_t total = 0;
+    for (auto &m : h->mem) {
+        int64_t n = reference_count(m);
+        if (m.s.max_molecules > 0 && n > m.s.max_molecules)
+            return fail(h, DSMC_ERR_CAPACITY, "more molecules than max_molecules_per_node (system.cpp:399-403)");
+        counts.push_back(n);
+        total += n;
+    }
+    dsmc_status st = seed_members(h, counts);
+    if (st == DSMC_OK && n_out) *n_out = total;
+    return st;
+}
+
+dsmc_status dsmc_initialize_counts(dsmc_handle *h, const int64_t *n_per_member) {
+    if (!h || !n_per_member) return DSMC_ERR_INVALID_ARGUMENT;
+    std::vector<int64_t> counts(n_per_member, n_per_member + h->mem.size());
+    for (auto c : counts)
+        if (c < 0) return DSMC_ERR_INVALID_ARGUMENT;
+    return seed_members(h, counts);
+}
+
+dsmc_status dsmc_initialize_fields(dsmc_handle *h, int member, const double *fields5, int64_t *n_out) {
+    (void)member; (void)fields5; (void)n_out;
+    return fail(h, DSMC_ERR_STATE, "field seeding not available in this build");
+}
+
+dsmc_status dsmc_load_state(dsmc_handle *h, const double *rv6, int64_t n) {
+    if (!h || (!rv6 && n > 0) || n < 0) return DSMC_ERR_INVALID_ARGUMENT;
+    if (h->mem.size() != 1) return fail(h, DSMC_ERR_STATE, "load_state needs a single-system handle");
+    MemberHost &m = h->mem[0];
+    if (m.s.max_molecules > 0 && n > m.s.max_molecules) return fail(h, DSMC_ERR_CAPACITY, "state larger than max_molecules");
+    dsmc_status st = alloc_particles(h, n);
+    if (st != DSMC_OK) return st;
+    std::vector<float> o[3], v[3];
+    std::vector<uint32_t> vx(n);
+    for (int a = 0; a < 3; a++) { o[a].resize(n); v[a].resize(n); }
+    for (int64_t p = 0; p < n; p++) {
+        int idx[3];
+        for (int a = 0; a < 3; a++) {
+            // voxel = (int)(r * (float)(n/L)) as grid.h:139-145 (origin 0)
+            double x = rv6[6 * p + a] * (double)m.dev.inv_h[a];
+            double f = std::floor(x);
+            int i = (int)f;
+            float off = (float)(x - f);
+            if (off >= 1.0f) { off = 0.0f; i += 1; }
+            int nn = m.dev.n[a];
+            i = ((i % nn) + nn) % nn;
+            idx[a] = i;
+            o[a][p] = off;
+            v[a][p] = (float)rv6[6 * p + 3 + a];
+        }
+        vx[p] = ((uint32_t)idx[0] << 20) | ((uint32_t)idx[1] << 10) | (uint32_t)idx[2];
+    }
+    int c = h->cur;
+    HIPCHK(h, hipMemcpy(h->ox[c], o[0].data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->oy[c], o[1].data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->oz[c], o[2].data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->vx[c], v[0].data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->vy[c], v[1].data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->vz[c], v[2].data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->vox[c], vx.data(), n * 4, hipMemcpyHostToDevice));
+    m.first = 0;
+    m.count = n;
+    std::vector<uint32_t> pm(n, 0u);
+    st = sort_unsorted(h, pm);
+    if (st == DSMC_OK) h->initialized = true;
+    return st;
+}
+
+dsmc_status dsmc_num_particles(dsmc_handle *h, int64_t *n) {
+    if (!h || !n) return DSMC_ERR_INVALID_ARGUMENT;
+    *n = h->N;
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_state(dsmc_handle *h, double *rv6) {
+    if (!h || (!rv6 && h->N > 0)) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->initialized) return fail(h, DSMC_ERR_STATE, "not initialized");
+    int64_t n = h->N;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::vector<float> o[3], v[3];
+    std::vector<uint32_t> vx(n);
+    for (int a = 0; a < 3; a++) { o[a].resize(n); v[a].resize(n); }
+    int c = h->cur;
+    HIPCHK(h, hipMemcpy(o[0].data(), h->ox[c], n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(o[1].data(), h->oy[c], n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(o[2].data(), h->oz[c], n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(v[0].data(), h->vx[c], n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(v[1].data(), h->vy[c], n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(v[2].data(), h->vz[c], n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(vx.data(), h->vox[c], n * 4, hipMemcpyDeviceToHost));
+    // member of each particle from the cell ranges
+    std::vector<uint32_t> st(h->ncells + 1);
+    HIPCHK(h, hipMemcpy(st.data(), h->d_start[h->scur], (h->ncells + 1) * 4, hipMemcpyDeviceToHost));
+    for (size_t mi = 0; mi < h->mem.size(); mi++) {
+        const MemberHost &m = h->mem[mi];
+        int64_t p0 = st[m.dev.cell0], p1 = st[m.dev.cell0 + m.dev.ncell];
+        double L[3] = {m.s.Lx, m.s.Ly, m.s.Lz};
+        for (int64_t p = p0; p < p1; p++) {
+            int idx[3] = {(int)(vx[p] >> 20), (int)((vx[p] >> 10) & 1023u), (int)(vx[p] & 1023u)};
+            for (int a = 0; a < 3; a++) {
+                double hh = L[a] / m.dev.n[a];
+                rv6[6 * p + a] = ((double)idx[a] + (double)o[a][p]) * hh;
+                rv6[6 * p + 3 + a] = v[a][p];
+            }
+        }
+    }
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_save_state_file(dsmc_handle *h, const char *path) {
+    if (!h || !path) return DSMC_ERR_INVALID_ARGUMENT;
+    if (h->N > 0x7fffffff) return fail(h, DSMC_ERR_CAPACITY, "state file stores an int count");
+    std::vector<double> rv(6 * h->N);
+    dsmc_status st = dsmc_get_state(h, rv.data());
+    if (st != DSMC_OK) return st;
+    std::ofstream f(path, std::ios::binary);     // dsmc_io.cpp:82-119
+    if (!f) return fail(h, DSMC_ERR_IO, "cannot open state file");
+    int n = (int)h->N;
+    f.write((const char *)&n, sizeof(int));
+    f.write((const char *)rv.data(), rv.size() * sizeof(double));
+    return f ? DSMC_OK : fail(h, DSMC_ERR_IO, "write failed");
+}
+
+dsmc_status dsmc_load_state_file(dsmc_handle *h, const char *path) {
+    if (!h || !path) return DSMC_ERR_INVALID_ARGUMENT;
+    std::ifstream f(path, std::ios::binary);     // dsmc_io.cpp:121-159
+    if (!f) return fail(h, DSMC_ERR_IO, "cannot open state file");
+    int n = 0;
+    f.read((char *)&n, sizeof(int));
+    if (!f || n < 0) return fail(h, DSMC_ERR_IO, "bad state file");
+    std::vector<double> rv(6 * (size_t)n);
+    f.read((char *)rv.data(), rv.size() * sizeof(double));
+    if (!f) return fail(h, DSMC_ERR_IO, "truncated state file");
+    return dsmc_load_state(h, rv.data(), n);
+}
+
+dsmc_status dsmc_step(dsmc_handle *h, int nsteps) {
+    if (!h || nsteps < 0) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->initialized) return fail(h, DSMC_ERR_STATE, "step before initialize (system.cpp:22)");
+    if (h->timing)
+        for (auto &t : h->timers) t.used = 0;
+    int done = 0;
+    while (done < nsteps) {
+        bool sample = h->sample_every > 0 && ((h->steps + 1) % h->sample_every) == 0;
+        bool graph_ok = h->use_graph && !h->timing && nsteps - done >= 2 &&
+                        (h->sample_every == 0 || h->sample_every == 1);
+        if (graph_ok) {
+            int par = h->cur;
+            if (!h->graph[par] || h->graph_sample[par] != sample) {
+                if (h->graph[par]) {
+                    (void)hipGraphExecDestroy(h->graph[par]);
+                    (void)hipGraphDestroy(h->graph_src[par]);
+                    h->graph[par] = nullptr;
+                    h->graph_src[par] = nullptr;
+                }
+                dsmc_status st = capture_graph(h, par, sample);
+                if (st != DSMC_OK) return st;
+            }
+            HIPCHK(h, hipGraphLaunch(h->graph[par], h->stream));
+            for (int k = 0; k < 2; k++) {
+                h->steps++;
+                h->t += h->mem[0].s.dt;
+                if (sample) h->nsamples++;
+            }
+            done += 2;
+            continue;
+        }
+        dsmc_status st = enqueue_step(h, sample);
+        if (st != DSMC_OK) return st;
+        h->steps++;
+        h->t += h->mem[0].s.dt;
+        if (sample) h->nsamples++;
+        done++;
+    }
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_synchronize(dsmc_handle *h) {
+    if (!h) return DSMC_ERR_INVALID_ARGUMENT;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_counters(dsmc_handle *h, dsmc_counters *c) {
+    if (!h || !c) return DSMC_ERR_INVALID_ARGUMENT;
+    unsigned long long v[CNT_NUM];
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(v, h->d_cnt, sizeof v, hipMemcpyDeviceToHost));
+    std::memset(c, 0, sizeof *c);
+    c->steps = h->steps;
+    c->t = h->t;
+    c->collisions = v[CNT_COLL];
+    c->wall_collisions = v[CNT_WALL];
+    c->count_periodic[0] = (int64_t)v[CNT_PER0];
+    c->count_periodic[1] = (int64_t)v[CNT_PER1];
+    c->count_periodic[2] = (int64_t)v[CNT_PER2];
+    c->reseeds = v[CNT_RESEED];
+    c->errors = v[CNT_ERROR];
+    c->candidates = v[CNT_CAND];
+    c->num_molecules = h->N;
+    c->max_cell_occupancy = h->max_occ;
+    c->overflow_cells = v[CNT_OVERFLOW];
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_set_wall_model(dsmc_handle *h, int model) {
+    if (!h || (model != DSMC_WALL_THERMAL && model != DSMC_WALL_SPECULAR)) return DSMC_ERR_INVALID_ARGUMENT;
+    std::vector<Member> ms;
+    for (auto &m : h->mem) {
+        m.dev.wall_model = model;
+        m.s.wall_model = model;
+        ms.push_back(m.dev);
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(h->d_members, ms.data(), ms.size() * sizeof(Member), hipMemcpyHostToDevice));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_set_sampling(dsmc_handle *h, int every) {
+    if (!h || every < 0) return DSMC_ERR_INVALID_ARGUMENT;
+    h->sample_every = every;
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_cell_moments(dsmc_handle *h, double *mom7, int64_t *nsamples) {
+    if (!h || !mom7) return DSMC_ERR_INVALID_ARGUMENT;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::vector<double> soa(7 * (size_t)h->ncells);
+    HIPCHK(h, hipMemcpy(soa.data(), h->d_mom, soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < h->ncells; c++)
+        for (int j = 0; j < 7; j++) mom7[7 * (size_t)c + j] = soa[(size_t)j * h->ncells + c];
+    if (nsamples) *nsamples = h->nsamples;
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_reset_cell_moments(dsmc_handle *h) {
+    if (!h) return DSMC_ERR_INVALID_ARGUMENT;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemset(h->d_mom, 0, 7 * (size_t)h->ncells * sizeof(double)));
+    h->nsamples = 0;
+    return DSMC_OK;
+}
+
+static dsmc_status totals_of(dsmc_handle *h, int buf, double *tot4) {
+    HIPCHK(h, launch_totals_partial(h->vx[buf], h->vy[buf], h->vz[buf], (uint32_t)h->N, h->d_partial,
+                                    TOT_BLOCKS, h->stream));
+    std::vector<double> part(4 * TOT_BLOCKS);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(part.data(), h->d_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < 4; j++) {
+        double s = 0;
+        for (int b = 0; b < TOT_BLOCKS; b++) s += part[4 * b + j];
+        tot4[j] = s;
+    }
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_totals(dsmc_handle *h, double *tot4) {
+    if (!h || !tot4) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->initialized) return fail(h, DSMC_ERR_STATE, "not initialized");
+    return totals_of(h, h->cur, tot4);
+}
+
+dsmc_status dsmc_get_collide_totals(dsmc_handle *h, double *tot8) {
+    if (!h || !tot8) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->totals) return fail(h, DSMC_ERR_STATE, "enable DSMC_OPT_TOTALS first");
+    std::vector<double> part(8 * TOT_BLOCKS);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(part.data(), h->d_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int half = 0; half < 2; half++)
+        for (int j = 0; j < 4; j++) {
+            double s = 0;
+            for (int b = 0; b < TOT_BLOCKS; b++) s += part[half * 4 * TOT_BLOCKS + 4 * b + j];
+            tot8[4 * half + j] = s;
+        }
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_num_cells(dsmc_handle *h, int64_t *nc) {
+    if (!h || !nc) return DSMC_ERR_INVALID_ARGUMENT;
+    *nc = h->ncells;
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_cells(dsmc_handle *h, double *coeff, double *vr_max, double *rest, int32_t *count) {
+    if (!h) return DSMC_ERR_INVALID_ARGUMENT;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    size_t n = h->ncells;
+    if (coeff) HIPCHK(h, hipMemcpy(coeff, h->d_coeff, n * 8, hipMemcpyDeviceToHost));
+    if (vr_max) HIPCHK(h, hipMemcpy(vr_max, h->d_vrmax, n * 8, hipMemcpyDeviceToHost));
+    if (rest) HIPCHK(h, hipMemcpy(rest, h->d_rest, n * 8, hipMemcpyDeviceToHost));
+    if (count) {
+        std::vector<uint32_t> st(n + 1);
+        HIPCHK(h, hipMemcpy(st.data(), h->d_start[h->scur], (n + 1) * 4, hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < n; c++) count[c] = (int32_t)(st[c + 1] - st[c]);
+    }
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_set_cells(dsmc_handle *h, const double *vr_max, const double *rest) {
+    if (!h) return DSMC_ERR_INVALID_ARGUMENT;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    size_t n = h->ncells;
+    if (vr_max) HIPCHK(h, hipMemcpy(h->d_vrmax, vr_max, n * 8, hipMemcpyHostToDevice));
+    if (rest) HIPCHK(h, hipMemcpy(h->d_rest, rest, n * 8, hipMemcpyHostToDevice));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_collide_stats(dsmc_handle *h, int32_t *pairs, int32_t *accepted) {
+    if (!h) return DSMC_ERR_INVALID_ARGUMENT;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (pairs) HIPCHK(h, hipMemcpy(pairs, h->d_pairs, h->ncells * 4, hipMemcpyDeviceToHost));
+    if (accepted) HIPCHK(h, hipMemcpy(accepted, h->d_acc, h->ncells * 4, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_set_option(dsmc_handle *h, int option, int64_t value) {
+    if (!h) return DSMC_ERR_INVALID_ARGUMENT;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    switch (option) {
+    case DSMC_OPT_PHASES:
+        if (value < 0 || value > 7) return DSMC_ERR_INVALID_ARGUMENT;
+        h->phases = (int)value;
+        drop_graphs(h);
+        return DSMC_OK;
+    case DSMC_OPT_RECORD:
+        h->record = value != 0;
+        drop_graphs(h);
+        if (h->record && h->N > 0 && !h->d_rec_perm) {
+            HIPCHK(h, dalloc(&h->d_rec_perm, h->N));
+            HIPCHK(h, dalloc(&h->d_rec_nhit, h->N));
+            HIPCHK(h, dalloc(&h->d_rec_last, h->N));
+        }
+        return DSMC_OK;
+    case DSMC_OPT_TOTALS:
+        h->totals = value != 0;
+        drop_graphs(h);
+        return DSMC_OK;
+    case DSMC_OPT_LDS_CAPACITY:
+        if (value < 0 || value > 4096) return DSMC_ERR_INVALID_ARGUMENT;
+        drop_graphs(h);
+        if (value == 0) {
+            h->lds_auto = true;
+            h->lds_cap = auto_lds_cap(h->max_occ);
+        } else {
+            h->lds_auto = false;
+            h->lds_cap = (uint32_t)((value + 63) / 64 * 64);
+        }
+        return DSMC_OK;
+    case DSMC_OPT_USE_GRAPH:
+        h->use_graph = value != 0;
+        if (!h->use_graph) drop_graphs(h);
+        return DSMC_OK;
+    case DSMC_OPT_TIMING:
+        h->timing = value != 0;
+        return DSMC_OK;
+    default:
+        return DSMC_ERR_INVALID_ARGUMENT;
+    }
+}
+
+dsmc_status dsmc_get_last_perm(dsmc_handle *h, int32_t *perm) {
+    if (!h || !perm) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->record || !h->d_rec_perm) return fail(h, DSMC_ERR_STATE, "enable DSMC_OPT_RECORD first");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(perm, h->d_rec_perm, h->N * 4, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_last_wall_hits(dsmc_handle *h, int32_t *nhits, int32_t *last_voxel) {
+    if (!h) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->record || !h->d_rec_nhit) return fail(h, DSMC_ERR_STATE, "enable DSMC_OPT_RECORD first");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (nhits) HIPCHK(h, hipMemcpy(nhits, h->d_rec_nhit, h->N * 4, hipMemcpyDeviceToHost));
+    if (last_voxel) HIPCHK(h, hipMemcpy(last_voxel, h->d_rec_last, h->N * 4, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_kernel_times(dsmc_handle *h, int *nkernels, char *names, int names_len, double *ms) {
+    if (!h || !nkernels) return DSMC_ERR_INVALID_ARGUMENT;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int cap = *nkernels;
+    *nkernels = (int)h->timers.size();
+    std::string all;
+    for (int i = 0; i < (int)h->timers.size(); i++) {
+        Timer &t = h->timers[i];
+        double sum = 0;
+        for (size_t k = 0; k < t.used; k++) {
+            float x = 0;
+            HIPCHK(h, hipEventElapsedTime(&x, t.ev[k].first, t.ev[k].second));
+            sum += x;
+        }
+        if (ms && i < cap) ms[i] = t.used ? sum / t.used : 0.0;
+        all += t.name;
+        all += ';';
+    }
+    if (names && names_len > 0) std::snprintf(names, (size_t)names_len, "%s", all.c_str());
+    return DSMC_OK;
+}
+
+}  // extern "C"

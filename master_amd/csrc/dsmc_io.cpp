@@ -1,0 +1,406 @@
+// dsmc_io.cpp -- host-side inputs of the DSMC step: the reference settings
+// file, the reference world file, and the voxel-geometry generator.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+#include "../../include/dsmc.h"
+#include "dsmc_internal.h"
+
+namespace dsmc {
+
+// ------------------------------------------------------------------ ran1
+// Numerical Recipes ran1 as the reference's Random::next_double
+// (random.cpp:43-68); the geometry generator consumes it in the reference's
+// order so its worlds are identical to the reference tool's.
+struct Ran1 {
+    long idum, iy, iv[32];
+    explicit Ran1(long seed) : idum(seed), iy(0) { std::memset(iv, 0, sizeof iv); }
+    double next() {
+        const long IA = 16807, IM = 2147483647, IQ = 127773, IR = 2836, NDIV = 1 + (IM - 1) / 32;
+        const double AM = 1.0 / IM, RNMX = 1.0 - 1.2e-7;
+        long k;
+        if (idum <= 0 || !iy) {
+            idum = (-idum < 1) ? 1 : -idum;
+            for (int j = 32 + 7; j >= 0; j--) {
+                k = idum / IQ;
+                idum = IA * (idum - k * IQ) - IR * k;
+                if (idum < 0) idum += IM;
+                if (j < 32) iv[j] = idum;
+            }
+            iy = iv[0];
+        }
+        k = idum / IQ;
+        idum = IA * (idum - k * IQ) - IR * k;
+        if (idum < 0) idum += IM;
+        int j = (int)(iy / NDIV);
+        iy = iv[j];
+        iv[j] = idum;
+        double t = AM * iy;
+        return t > RNMX ? RNMX : t;
+    }
+};
+
+// ------------------------------------------------------------------ settings
+static std::string trim(const std::string &s) {      // cutil.cpp:82-93 (spaces only)
+    size_t a = s.find_first_not_of(' ');
+    if (a == std::string::npos) return "";
+    size_t b = s.find_last_not_of(' ');
+    return s.substr(a, b - a + 1);
+}
+
+bool parse_ini(const char *path, std::vector<std::pair<std::string, std::string>> &items) {
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string line;
+    while (std::getline(f, line)) {                  // cinifile.h:28-54
+        size_t cr = line.find('\r');
+        if (cr != std::string::npos) line.erase(cr, 1);
+        std::vector<std::string> tok;
+        size_t pos = line.find_first_not_of('=');
+        while (pos != std::string::npos) {
+            size_t end = line.find_first_of('=', pos);
+            tok.push_back(line.substr(pos, end == std::string::npos ? std::string::npos : end - pos));
+            pos = end == std::string::npos ? end : line.find_first_not_of('=', end);
+        }
+        if (tok.size() == 2) items.push_back({trim(tok[0]), trim(tok[1])});
+    }
+    return true;
+}
+
+static const std::string *find(const std::vector<std::pair<std::string, std::string>> &items,
+                               const char *key) {
+    for (auto &kv : items)
+        if (kv.first == key) return &kv.second;
+    return nullptr;
+}
+
+}  // namespace dsmc
+
+using namespace dsmc;
+
+extern "C" dsmc_status dsmc_settings_from_ini(const char *path, dsmc_settings *out, char *world_dir,
+                                              int world_dir_len) {
+    if (!path || !out) return DSMC_ERR_INVALID_ARGUMENT;
+    std::vector<std::pair<std::string, std::string>> items;
+    if (!parse_ini(path, items)) return DSMC_ERR_IO;
+    auto getd = [&](const char *k, double &v) -> bool {
+        const std::string *s = find(items, k);
+        if (!s) return false;
+        v = std::strtod(s->c_str(), nullptr);
+        return true;
+    };
+    double v;
+    dsmc_settings s;
+    std::memset(&s, 0, sizeof s);
+    // settings.cpp:6-42; unit conversion of system.cpp:253-266
+    const double T0 = 1.65088e-21 / 1.3806503e-23;
+    const double L0 = 1e-6;
+    const double nd0 = 1.0 / (L0 * L0 * L0);
+#define REQ(k, expr) do { if (!getd(k, v)) return DSMC_ERR_CONFIG; expr; } while (0)
+    REQ("Lx", s.Lx = v); REQ("Ly", s.Ly = v); REQ("Lz", s.Lz = v);
+    REQ("cells_x", s.cells_x = (int)v); REQ("cells_y", s.cells_y = (int)v); REQ("cells_z", s.cells_z = (int)v);
+    REQ("dt", s.dt = v);
+    REQ("mass", s.mass = v);
+    REQ("temperature", s.temperature = v / T0);
+    REQ("wall_temperature", s.wall_temperature = v / T0);
+    REQ("gravity", s.gravity = v);
+    REQ("flow_direction", s.flow_direction = (int)v);
+    REQ("density", s.density = v / nd0);
+    REQ("diam", s.diam = v);
+    REQ("atoms_per_molecule", s.atoms_per_molecule = (int)v);
+    REQ("seed", s.seed = (int)v);
+    REQ("max_molecules_per_node", s.max_molecules = (int64_t)(int)v);
+#undef REQ
+    const std::string *model = find(items, "surface_interaction_model");
+    if (!model) return DSMC_ERR_CONFIG;
+    if (*model == "thermal") s.wall_model = DSMC_WALL_THERMAL;
+    else if (*model == "specular") s.wall_model = DSMC_WALL_SPECULAR;
+    else return DSMC_ERR_CONFIG;   // cercignani_lampis / maxwell: not on this path
+    if (world_dir && world_dir_len > 0) {
+        const std::string *w = find(items, "world");
+        std::snprintf(world_dir, (size_t)world_dir_len, "%s", w ? w->c_str() : "");
+    }
+    *out = s;
+    return DSMC_OK;
+}
+
+// ------------------------------------------------------------------ world file
+extern "C" dsmc_status dsmc_world_read(const char *path, dsmc_world *out) {
+    if (!path || !out) return DSMC_ERR_INVALID_ARGUMENT;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return DSMC_ERR_IO;
+    float pg = 0, pl = 0;
+    uint32_t g[3], h[3];
+    f.read((char *)&pg, 4);
+    f.read((char *)&pl, 4);
+    f.read((char *)g, 12);
+    f.read((char *)h, 12);
+    if (!f) return DSMC_ERR_IO;
+    // one-rank file: the halo grid is the periodic grid tiled 3x per axis
+    if (h[0] != 3 * g[0] || h[1] != 3 * g[1] || h[2] != 3 * g[2]) return DSMC_ERR_CONFIG;
+    size_t hn = (size_t)h[0] * h[1] * h[2];
+    std::vector<uint8_t> ht(hn);
+    std::vector<float> hv(3 * hn);
+    f.read((char *)ht.data(), hn);
+    size_t n = (size_t)g[0] * g[1] * g[2];
+    uint8_t *types = (uint8_t *)std::malloc(n);
+    float *vecs[3];
+    for (int q = 0; q < 3; q++) vecs[q] = (float *)std::malloc(12 * n);
+    for (size_t i = 0; i < g[0]; i++)
+        for (size_t j = 0; j < g[1]; j++)
+            for (size_t k = 0; k < g[2]; k++)
+                types[(i * g[1] + j) * g[2] + k] = ht[((i + g[0]) * h[1] + (j + g[1])) * h[2] + k + g[2]];
+    for (int q = 0; q < 3; q++) {
+        f.read((char *)hv.data(), 12 * hn);
+        if (!f) {
+            std::free(types);
+            for (int r = 0; r < 3; r++) std::free(vecs[r]);
+            return DSMC_ERR_IO;
+        }
+        for (size_t i = 0; i < g[0]; i++)
+            for (size_t j = 0; j < g[1]; j++)
+                for (size_t k = 0; k < g[2]; k++) {
+                    size_t src = ((i + g[0]) * h[1] + (j + g[1])) * h[2] + k + g[2];
+                    size_t dst = (i * g[1] + j) * g[2] + k;
+                    for (int a = 0; a < 3; a++) vecs[q][3 * dst + a] = hv[3 * src + a];
+                }
+    }
+    out->nx = g[0]; out->ny = g[1]; out->nz = g[2];
+    out->types = types;
+    out->normals = vecs[0];
+    out->tangents1 = vecs[1];
+    out->tangents2 = vecs[2];
+    out->porosity = pl;
+    return DSMC_OK;
+}
+
+extern "C" void dsmc_world_free(dsmc_world *w) {
+    if (!w) return;
+    std::free((void *)w->types);
+    std::free((void *)w->normals);
+    std::free((void *)w->tangents1);
+    std::free((void *)w->tangents2);
+    w->types = nullptr; w->normals = w->tangents1 = w->tangents2 = nullptr;
+}
+
+extern "C" dsmc_status dsmc_world_write(const char *path, const dsmc_world *w) {
+    if (!path || !w || !w->types) return DSMC_ERR_INVALID_ARGUMENT;
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return DSMC_ERR_IO;
+    uint32_t g[3] = {w->nx, w->ny, w->nz}, h[3] = {3 * w->nx, 3 * w->ny, 3 * w->nz};
+    float por = w->porosity;
+    f.write((const char *)&por, 4);
+    f.write((const char *)&por, 4);
+    f.write((const char *)g, 12);
+    f.write((const char *)h, 12);
+    // complexgeometry.cpp:251-281: di in [-n, 2n) per axis, z fastest
+    auto src = [&](long di, long dj, long dk) {
+        long i = (di + 10L * g[0]) % g[0], j = (dj + 10L * g[1]) % g[1], k = (dk + 10L * g[2]) % g[2];
+        return (size_t)((i * g[1] + j) * g[2] + k);
+    };
+    std::vector<uint8_t> row(h[2]);
+    for (long di = -(long)g[0]; di < 2L * g[0]; di++)
+        for (long dj = -(long)g[1]; dj < 2L * g[1]; dj++) {
+            for (long dk = -(long)g[2]; dk < 2L * g[2]; dk++) row[dk + g[2]] = w->types[src(di, dj, dk)];
+            f.write((const char *)row.data(), h[2]);
+        }
+    const float *vec[3] = {w->normals, w->tangents1, w->tangents2};
+    std::vector<float> vrow(3 * h[2]);
+    for (int q = 0; q < 3; q++)
+        for (long di = -(long)g[0]; di < 2L * g[0]; di++)
+            for (long dj = -(long)g[1]; dj < 2L * g[1]; dj++) {
+                for (long dk = -(long)g[2]; dk < 2L * g[2]; dk++)
+                    for (int a = 0; a < 3; a++) vrow[3 * (dk + g[2]) + a] = vec[q][3 * src(di, dj, dk) + a];
+                f.write((const char *)vrow.data(), 12 * h[2]);
+            }
+    return f ? DSMC_OK : DSMC_ERR_IO;
+}
+
+// ------------------------------------------------------------------ geometry
+namespace dsmc {
+
+// calculate_normals (complexgeometry.cpp:928-999): 3x3x3 wall-offset sum,
+// re-summed over +-nna when non-zero, normalised; walls with a zero normal
+// that are not fully enclosed are cleared (in scan order, as the reference).
+static void geom_normals(int nx, int ny, int nz, int nna, uint8_t *t, float *nrm) {
+    size_t nv = (size_t)nx * ny * nz;
+    std::memset(nrm, 0, 12 * nv);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+                bool any_wall = false, all_wall = true;
+                size_t idx = ((size_t)i * ny + j) * nz + k;
+                float *n = nrm + 3 * idx;
+                for (int di = -1; di <= 1; di++)
+                    for (int dj = -1; dj <= 1; dj++)
+                        for (int dk = -1; dk <= 1; dk++) {
+                            size_t q = ((size_t)((i + di + nx) % nx) * ny + (j + dj + ny) % ny) * nz + (k + dk + nz) % nz;
+                            if (t[q] > 0) any_wall = true;
+                            if (t[q] == 0) all_wall = false;
+                            n[0] -= (float)(t[q] * di);
+                            n[1] -= (float)(t[q] * dj);
+                            n[2] -= (float)(t[q] * dk);
+                        }
+                double nsq = (double)(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                if (nsq > 0) {
+                    n[0] = n[1] = n[2] = 0;
+                    for (int di = -nna; di <= nna; di++)
+                        for (int dj = -nna; dj <= nna; dj++)
+                            for (int dk = -nna; dk <= nna; dk++) {
+                                size_t q = ((size_t)((i + di + nx) % nx) * ny + (j + dj + ny) % ny) * nz + (k + dk + nz) % nz;
+                                n[0] -= (float)(t[q] * di);
+                                n[1] -= (float)(t[q] * dj);
+                                n[2] -= (float)(t[q] * dk);
+                            }
+                }
+                double norm = (double)std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);  // float overload
+                if (norm > 0) {
+                    n[0] = (float)(n[0] / norm);
+                    n[1] = (float)(n[1] / norm);
+                    n[2] = (float)(n[2] / norm);
+                } else if (!any_wall || !all_wall) {
+                    t[idx] = 0;
+                }
+            }
+}
+
+// calculate_tangents (1001-1046) and find_boundary_points (1048-1063).
+static void geom_tangents_boundary(size_t nv, uint8_t *t, const float *nrm, float *t1, float *t2) {
+    Ran1 rnd(-1);
+    for (size_t idx = 0; idx < nv; idx++) {
+        const float *n = nrm + 3 * idx;
+        float *a = t1 + 3 * idx, *b = t2 + 3 * idx;
+        a[0] = (float)rnd.next();
+        a[1] = (float)rnd.next();
+        a[2] = (float)rnd.next();
+        double dot = (double)(n[0] * a[0] + n[1] * a[1] + n[2] * a[2]);
+        for (int q = 0; q < 3; q++) a[q] = (float)((double)a[q] - (double)n[q] * dot);
+        double norm = (double)std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        if (norm > 0)
+            for (int q = 0; q < 3; q++) a[q] = (float)(a[q] / norm);
+        b[0] = a[1] * n[2] - a[2] * n[1];
+        b[1] = a[2] * n[0] - a[0] * n[2];
+        b[2] = a[0] * n[1] - a[1] * n[0];
+        norm = (double)std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        if (norm > 0)
+            for (int q = 0; q < 3; q++) b[q] = (float)(b[q] / norm);
+    }
+    for (size_t idx = 0; idx < nv; idx++) {
+        const float *n = nrm + 3 * idx;
+        double nn = (double)(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (t[idx] > 0 && nn > 0) t[idx] = 2;
+    }
+}
+
+static float geom_porosity(size_t nv, const uint8_t *t) {   // 173-179
+    int sum = 0;
+    for (size_t i = 0; i < nv; i++) sum += t[i] == 0;
+    return (float)((double)sum / nv);
+}
+
+static void finish_world(int nx, int ny, int nz, int nna, uint8_t *types, dsmc_world *out) {
+    size_t nv = (size_t)nx * ny * nz;
+    float *nrm = (float *)std::malloc(12 * nv), *t1 = (float *)std::malloc(12 * nv),
+          *t2 = (float *)std::malloc(12 * nv);
+    geom_normals(nx, ny, nz, nna, types, nrm);
+    geom_tangents_boundary(nv, types, nrm, t1, t2);
+    out->nx = nx; out->ny = ny; out->nz = nz;
+    out->types = types;
+    out->normals = nrm;
+    out->tangents1 = t1;
+    out->tangents2 = t2;
+    out->porosity = geom_porosity(nv, types);
+}
+
+}  // namespace dsmc
+
+// complexgeometry.cpp:401-513 with spheres_type=1, inverted=true,
+// prevent_overlap=false: spheres centred in the first octant until its
+// porosity drops below the target, then mirrored (make_periodic, 853-871).
+extern "C" dsmc_status dsmc_geometry_packed_spheres(int n, double radius_vox, double porosity, int seed,
+                                                    int nna, dsmc_world *out) {
+    if (n <= 1 || n > 1024 || !out || radius_vox <= 0) return DSMC_ERR_INVALID_ARGUMENT;
+    int nx = n, ny = n, nz = n;
+    size_t nv = (size_t)nx * ny * nz;
+    uint8_t *t = (uint8_t *)std::calloc(nv, 1);
+    double radius = radius_vox / n, r2 = radius * radius;
+    float vsx = (float)(1.0 / nx), vsy = (float)(1.0 / ny), vsz = (float)(1.0 / nz);
+    int hx = nx / 2, hy = ny / 2, hz = nz / 2;
+    Ran1 rnd(-std::abs(seed));
+    bool finished = false;
+    long guard = 0;
+    while (!finished) {
+        double x0 = rnd.next() * 0.5, y0 = rnd.next() * 0.5, z0 = rnd.next() * 0.5;
+        int empty = 0;
+        for (int i = 0; i < hx; i++)
+            for (int j = 0; j < hy; j++)
+                for (int k = 0; k < hz; k++) {
+                    double x = i / (double)nx + vsx / 2.0, y = j / (double)ny + vsy / 2.0,
+                           z = k / (double)nz + vsz / 2.0;
+                    double dx = x - x0, dy = y - y0, dz = z - z0;
+                    size_t idx = ((size_t)i * ny + j) * nz + k;
+                    if (dx * dx + dy * dy + dz * dz < r2) t[idx] = 1;
+                    empty += t[idx] == 0;
+                }
+        double por = (double)empty / (hx * hy * hz);
+        finished = por < porosity;
+        if (++guard > 10000000) { std::free(t); return DSMC_ERR_CONFIG; }
+    }
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+                int gi = i < nx / 2 ? i : nx - i - 1, gj = j < ny / 2 ? j : ny - j - 1,
+                    gk = k < nz / 2 ? k : nz - k - 1;
+                t[((size_t)i * ny + j) * nz + k] = t[((size_t)gi * ny + gj) * nz + gk];
+            }
+    finish_world(nx, ny, nz, nna, t, out);
+    return DSMC_OK;
+}
+
+// Polydisperse periodic spheres (no reference counterpart: the reference
+// generator supports one radius).  Sphere s takes its centre and radius from
+// Philox4x32-10(ctr = {7, 0, s, 0}, key = seed).
+extern "C" dsmc_status dsmc_geometry_poly_spheres(uint32_t nx, uint32_t ny, uint32_t nz, double rmin,
+                                                  double rmax, double porosity, uint64_t seed, int nna,
+                                                  dsmc_world *out) {
+    if (!out || nx < 2 || ny < 2 || nz < 2 || nx > 1024 || ny > 1024 || nz > 1024 || rmin <= 0 || rmax < rmin)
+        return DSMC_ERR_INVALID_ARGUMENT;
+    size_t nv = (size_t)nx * ny * nz;
+    uint8_t *t = (uint8_t *)std::calloc(nv, 1);
+    long empty = (long)nv;
+    double l0 = std::log(rmin), l1 = std::log(rmax);
+    for (uint32_t s = 0; (double)empty / (double)nv >= porosity; s++) {
+        uint32_t o[4];
+        host_philox(7u, 0u, s, 0u, (uint32_t)seed, (uint32_t)(seed >> 32), o);
+        double cx = host_u01(o[0]) * nx, cy = host_u01(o[1]) * ny, cz = host_u01(o[2]) * nz;
+        double rad = std::exp(l0 + (l1 - l0) * host_u01(o[3]));
+        double rr = rad * rad;
+        int i0 = (int)std::floor(cx - rad) - 1, i1 = (int)std::ceil(cx + rad) + 1;
+        int j0 = (int)std::floor(cy - rad) - 1, j1 = (int)std::ceil(cy + rad) + 1;
+        int k0 = (int)std::floor(cz - rad) - 1, k1 = (int)std::ceil(cz + rad) + 1;
+        for (int i = i0; i <= i1; i++) {
+            double dx = (i + 0.5) - cx;
+            int ii = ((i % (int)nx) + (int)nx) % (int)nx;
+            for (int j = j0; j <= j1; j++) {
+                double dy = (j + 0.5) - cy;
+                int jj = ((j % (int)ny) + (int)ny) % (int)ny;
+                for (int k = k0; k <= k1; k++) {
+                    double dz = (k + 0.5) - cz;
+                    int kk = ((k % (int)nz) + (int)nz) % (int)nz;
+                    if (dx * dx + dy * dy + dz * dz < rr) {
+                        size_t idx = ((size_t)ii * ny + jj) * nz + kk;
+                        if (!t[idx]) { t[idx] = 1; empty--; }
+                    }
+                }
+            }
+        }
+        if (s > 100000000u) { std::free(t); return DSMC_ERR_CONFIG; }
+    }
+    finish_world((int)nx, (int)ny, (int)nz, nna, t, out);
+    return DSMC_OK;
+}

@@ -1,0 +1,96 @@
+// dsmc_kernels.h -- kernel argument blocks and launchers shared by the host
+// System (dsmc_host.cpp) and the kernels (dsmc_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "dsmc_device.h"
+
+namespace dsmc {
+
+static constexpr int MAX_TILE = 1000;          // (b+2)^3 voxel tile, b <= 8
+static constexpr int SCAN_THREADS = 1024;
+static constexpr int SCAN_TILE = 4096;         // cells per scan block
+
+enum CounterSlot {
+    CNT_WALL = 0, CNT_RESEED, CNT_ERROR, CNT_PER0, CNT_PER1, CNT_PER2,
+    CNT_COLL, CNT_CAND, CNT_OVERFLOW, CNT_NUM
+};
+
+struct MoveArgs {
+    float *ox, *oy, *oz, *vx, *vy, *vz;
+    uint32_t *vox;
+    const uint32_t *start;
+    uint32_t *leave, *arrive;
+    uint32_t *mv_idx, *mv_cell, *mv_count;
+    const Member *members;
+    const uint32_t *cell_member;
+    const uint8_t *types;
+    const float *frames;
+    const uint32_t *step;
+    unsigned long long *counters;
+    int32_t *rec_nhit, *rec_last;       // optional recording (per index before the step)
+    uint32_t cell_begin;
+};
+
+struct ClassifyArgs {
+    const uint32_t *vox;
+    const uint32_t *part_member;
+    const Member *members;
+    uint32_t *arrive, *mv_idx, *mv_cell, *mv_count;
+    uint32_t n;
+};
+
+struct ScanArgs {
+    const uint32_t *start, *leave, *arrive;
+    uint32_t *start_new, *bstart, *bfill;
+    unsigned long long *block_sums;
+    uint32_t ncells, n_blocks;
+};
+
+struct BucketArgs {
+    const uint32_t *mv_idx, *mv_cell, *mv_count, *bstart;
+    uint32_t *bfill, *bucket;
+};
+
+struct GatherArgs {
+    const float *ox, *oy, *oz, *vx, *vy, *vz;
+    const uint32_t *vox;
+    float *ox2, *oy2, *oz2, *vx2, *vy2, *vz2;
+    uint32_t *vox2;
+    const uint32_t *start, *start_new, *bstart, *bucket;
+    const Member *members;
+    const uint32_t *cell_member;
+    const double *coeff;
+    double *vr_max, *rest;
+    int32_t *stat_pairs, *stat_acc;
+    double *moments;                    // 7 x ncells (SoA)
+    const uint32_t *step;
+    unsigned long long *counters;
+    int32_t *rec_perm;
+    uint32_t ncells, cell_begin, lds_cap;
+    int do_collide, do_sample;
+};
+
+struct SeedArgs {
+    float *ox, *oy, *oz, *vx, *vy, *vz;
+    uint32_t *vox;
+    const uint32_t *part_member;
+    const uint32_t *member_first;
+    const Member *members;
+    const uint8_t *types;
+    unsigned long long *counters;
+    uint32_t n;
+};
+
+hipError_t launch_move(const MoveArgs &a, uint32_t ncells, hipStream_t s);
+hipError_t launch_classify_all(const ClassifyArgs &a, hipStream_t s);
+hipError_t launch_scan(const ScanArgs &a, hipStream_t s);
+hipError_t launch_bucket(const BucketArgs &a, hipStream_t s);
+hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, hipStream_t s);
+hipError_t launch_seed(const SeedArgs &a, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_begin_step(uint32_t *step, uint32_t *mv_count, hipStream_t s);
+hipError_t launch_totals_partial(const float *vx, const float *vy, const float *vz, uint32_t n,
+                                 double *partial, uint32_t nblocks, hipStream_t s);
+
+}  // namespace dsmc

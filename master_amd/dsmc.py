@@ -1,0 +1,350 @@
+"""Python view of the C ABI in include/dsmc.h (master_amd/libdsmc.so).
+
+`System` mirrors the reference's System (code/base_code/program/dsmc/
+system.h:29-120): `initialize()` (system.cpp:241-382), `step()`
+(system.cpp:22-47), and the public state the reference's main loop and
+samplers read (r, v, steps, t, collisions, num_molecules_local,
+count_periodic, the surface collider's num_collisions).  There is no CPU
+fallback: constructing a System without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import config as _config
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdsmc.so")
+P = ctypes.c_void_p
+D = ctypes.c_double
+I64 = ctypes.c_int64
+
+WALL_THERMAL, WALL_SPECULAR = 0, 1
+OPT_PHASES, OPT_RECORD, OPT_TOTALS, OPT_LDS_CAPACITY, OPT_USE_GRAPH, OPT_TIMING = 1, 2, 3, 4, 5, 6
+PHASE_MOVE, PHASE_SORT, PHASE_COLLIDE = 1, 2, 4
+
+_STATUS = {0: "OK", 1: "invalid argument", 2: "configuration", 3: "capacity", 4: "HIP runtime",
+           5: "I/O", 6: "state", 7: "numeric"}
+
+_lib = None
+
+
+class DsmcError(RuntimeError):
+    pass
+
+
+class Settings(ctypes.Structure):
+    """dsmc_settings: the reference Settings after System::initialize's unit conversion."""
+    _fields_ = [("Lx", D), ("Ly", D), ("Lz", D),
+                ("cells_x", ctypes.c_int), ("cells_y", ctypes.c_int), ("cells_z", ctypes.c_int),
+                ("dt", D), ("mass", D), ("temperature", D), ("wall_temperature", D), ("gravity", D),
+                ("flow_direction", ctypes.c_int), ("density", D), ("diam", D),
+                ("atoms_per_molecule", ctypes.c_int), ("seed", ctypes.c_int), ("wall_model", ctypes.c_int),
+                ("max_molecules", I64)]
+
+
+class World(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("nz", ctypes.c_uint32),
+                ("types", P), ("normals", P), ("tangents1", P), ("tangents2", P), ("porosity", ctypes.c_float)]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("steps", I64), ("t", D), ("collisions", ctypes.c_uint64), ("wall_collisions", ctypes.c_uint64),
+                ("count_periodic", I64 * 3), ("reseeds", ctypes.c_uint64), ("errors", ctypes.c_uint64),
+                ("candidates", ctypes.c_uint64), ("num_molecules", I64), ("max_cell_occupancy", I64),
+                ("overflow_cells", ctypes.c_uint64)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DsmcError(f"{LIB_PATH} is not built: run __graft_entry__.build() (no CPU fallback exists)")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.dsmc_last_error.restype = ctypes.c_char_p
+        _lib.dsmc_last_error.argtypes = [P]
+        for name in ("dsmc_create", "dsmc_create_ensemble", "dsmc_initialize", "dsmc_initialize_counts",
+                     "dsmc_load_state", "dsmc_get_state", "dsmc_step", "dsmc_synchronize", "dsmc_get_counters",
+                     "dsmc_set_wall_model", "dsmc_set_sampling", "dsmc_get_cell_moments",
+                     "dsmc_reset_cell_moments", "dsmc_get_totals", "dsmc_num_cells", "dsmc_get_cells",
+                     "dsmc_set_cells", "dsmc_get_collide_stats", "dsmc_set_option", "dsmc_get_last_perm",
+                     "dsmc_get_last_wall_hits", "dsmc_get_collide_totals", "dsmc_get_kernel_times",
+                     "dsmc_num_particles", "dsmc_settings_from_ini", "dsmc_world_read",
+                     "dsmc_geometry_packed_spheres", "dsmc_geometry_poly_spheres", "dsmc_world_write",
+                     "dsmc_save_state_file", "dsmc_load_state_file", "dsmc_initialize_fields"):
+            getattr(_lib, name).restype = ctypes.c_int
+        _lib.dsmc_destroy.restype = None
+        _lib.dsmc_world_free.restype = None
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(P)
+
+
+def _check(st, h=None, what=""):
+    if st != 0:
+        msg = ""
+        if h:
+            msg = lib().dsmc_last_error(h).decode(errors="replace")
+        raise DsmcError(f"{what}: {_STATUS.get(st, st)} error {msg}")
+
+
+# ------------------------------------------------------------------ settings/world
+def settings_from_ini(ini: dict) -> Settings:
+    """dsmc_settings from reference ini keys (SI temperature and density),
+    converted as System::initialize does (system.cpp:253-266)."""
+    s = Settings()
+    s.Lx, s.Ly, s.Lz = float(ini["Lx"]), float(ini["Ly"]), float(ini["Lz"])
+    s.cells_x, s.cells_y, s.cells_z = int(float(ini["cells_x"])), int(float(ini["cells_y"])), int(float(ini["cells_z"]))
+    s.dt = float(ini["dt"])
+    s.mass = float(ini["mass"])
+    s.temperature = float(ini["temperature"]) / _config.T0_TEMP
+    s.wall_temperature = float(ini["wall_temperature"]) / _config.T0_TEMP
+    s.gravity = float(ini["gravity"])
+    s.flow_direction = int(float(ini["flow_direction"]))
+    s.density = float(ini["density"]) / _config.NUMBER_DENSITY0
+    s.diam = float(ini["diam"])
+    s.atoms_per_molecule = int(float(ini["atoms_per_molecule"]))
+    s.seed = int(float(ini["seed"]))
+    model = ini["surface_interaction_model"]
+    if model not in ("thermal", "specular"):
+        raise DsmcError(f"surface_interaction_model {model!r} is not on this path")
+    s.wall_model = WALL_SPECULAR if model == "specular" else WALL_THERMAL
+    s.max_molecules = int(float(ini.get("max_molecules_per_node", 0)))
+    return s
+
+
+def settings_from_ini_file(path) -> tuple[Settings, str]:
+    s = Settings()
+    buf = ctypes.create_string_buffer(4096)
+    _check(lib().dsmc_settings_from_ini(os.fsencode(path), ctypes.byref(s), buf, 4096), what="settings_from_ini")
+    return s, buf.value.decode()
+
+
+class WorldData:
+    """A voxel world held as numpy arrays (z-fastest), kept alive while a
+    handle might read it."""
+
+    def __init__(self, n, types, normals, t1, t2, porosity):
+        self.n = tuple(int(x) for x in n)
+        self.types = np.ascontiguousarray(types, dtype=np.uint8).reshape(-1)
+        self.normals = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1)
+        self.t1 = np.ascontiguousarray(t1, dtype=np.float32).reshape(-1)
+        self.t2 = np.ascontiguousarray(t2, dtype=np.float32).reshape(-1)
+        self.porosity = float(porosity)
+
+    @classmethod
+    def from_dict(cls, w):
+        return cls(w["n"], w["types"], w["normals"], w["t1"], w["t2"], w["porosity"])
+
+    def as_dict(self):
+        return {"n": self.n, "types": self.types, "normals": self.normals, "t1": self.t1, "t2": self.t2,
+                "porosity": self.porosity}
+
+    def c_struct(self) -> World:
+        w = World()
+        w.nx, w.ny, w.nz = self.n
+        w.types, w.normals = _ptr(self.types), _ptr(self.normals)
+        w.tangents1, w.tangents2 = _ptr(self.t1), _ptr(self.t2)
+        w.porosity = self.porosity
+        return w
+
+    @classmethod
+    def _from_c(cls, w: World):
+        nv = w.nx * w.ny * w.nz
+        get = lambda p, n, t: np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(t)), shape=(n,)).copy()
+        out = cls((w.nx, w.ny, w.nz), get(w.types, nv, ctypes.c_uint8), get(w.normals, 3 * nv, ctypes.c_float),
+                  get(w.tangents1, 3 * nv, ctypes.c_float), get(w.tangents2, 3 * nv, ctypes.c_float), w.porosity)
+        lib().dsmc_world_free(ctypes.byref(w))
+        return out
+
+    @classmethod
+    def read(cls, path):
+        w = World()
+        _check(lib().dsmc_world_read(os.fsencode(path), ctypes.byref(w)), what="world_read")
+        return cls._from_c(w)
+
+    def write(self, path):
+        w = self.c_struct()
+        _check(lib().dsmc_world_write(os.fsencode(path), ctypes.byref(w)), what="world_write")
+
+    @classmethod
+    def packed_spheres(cls, n, radius_vox, porosity, seed, nna=1):
+        w = World()
+        _check(lib().dsmc_geometry_packed_spheres(int(n), D(radius_vox), D(porosity), int(seed), int(nna),
+                                                  ctypes.byref(w)), what="packed_spheres")
+        return cls._from_c(w)
+
+    @classmethod
+    def poly_spheres(cls, shape, rmin, rmax, porosity, seed, nna=1):
+        w = World()
+        nx, ny, nz = shape
+        _check(lib().dsmc_geometry_poly_spheres(ctypes.c_uint32(nx), ctypes.c_uint32(ny), ctypes.c_uint32(nz),
+                                                D(rmin), D(rmax), D(porosity), ctypes.c_uint64(seed), int(nna),
+                                                ctypes.byref(w)), what="poly_spheres")
+        return cls._from_c(w)
+
+
+# ------------------------------------------------------------------ System
+class System:
+    """One handle: a single system, or an ensemble of independent members."""
+
+    def __init__(self, settings, worlds):
+        if isinstance(settings, (Settings, dict)):
+            settings, worlds = [settings], [worlds]
+        self.settings = [s if isinstance(s, Settings) else settings_from_ini(s) for s in settings]
+        self.worlds = [w if isinstance(w, WorldData) else WorldData.from_dict(w) for w in worlds]
+        n = len(self.settings)
+        sarr = (Settings * n)(*self.settings)
+        warr = (World * n)(*[w.c_struct() for w in self.worlds])
+        h = P()
+        _check(lib().dsmc_create_ensemble(n, sarr, warr, ctypes.byref(h)), what="dsmc_create")
+        self.h = h
+        nc = I64()
+        self._call("dsmc_num_cells", ctypes.byref(nc))
+        self.ncells = nc.value
+
+    def _call(self, name, *args):
+        _check(getattr(lib(), name)(self.h, *args), self.h, name)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dsmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- System::initialize / setup_molecules
+    def initialize(self, counts=None):
+        if counts is None:
+            n = I64()
+            self._call("dsmc_initialize", ctypes.byref(n))
+            return n.value
+        c = np.ascontiguousarray(counts, dtype=np.int64)
+        self._call("dsmc_initialize_counts", _ptr(c))
+        return int(c.sum())
+
+    def load_state(self, rv6):
+        rv6 = np.ascontiguousarray(rv6, dtype=np.float64)
+        self._call("dsmc_load_state", _ptr(rv6), I64(len(rv6)))
+
+    def step(self, n=1):
+        self._call("dsmc_step", ctypes.c_int(n))
+
+    def synchronize(self):
+        self._call("dsmc_synchronize")
+
+    @property
+    def n(self):
+        x = I64()
+        self._call("dsmc_num_particles", ctypes.byref(x))
+        return x.value
+
+    def state(self):
+        out = np.zeros((self.n, 6))
+        self._call("dsmc_get_state", _ptr(out))
+        return out
+
+    @property
+    def r(self):
+        return self.state()[:, :3]
+
+    @property
+    def v(self):
+        return self.state()[:, 3:]
+
+    def counters(self):
+        c = Counters()
+        self._call("dsmc_get_counters", ctypes.byref(c))
+        return {f: (list(getattr(c, f)) if f == "count_periodic" else getattr(c, f)) for f, _ in Counters._fields_}
+
+    def set_option(self, opt, value):
+        self._call("dsmc_set_option", ctypes.c_int(opt), I64(int(value)))
+
+    def set_wall_model(self, model):
+        self._call("dsmc_set_wall_model", ctypes.c_int(model))
+
+    def set_sampling(self, every):
+        self._call("dsmc_set_sampling", ctypes.c_int(every))
+
+    def moments(self):
+        m = np.zeros((self.ncells, 7))
+        ns = I64()
+        self._call("dsmc_get_cell_moments", _ptr(m), ctypes.byref(ns))
+        return m, ns.value
+
+    def reset_moments(self):
+        self._call("dsmc_reset_cell_moments")
+
+    def totals(self):
+        t = np.zeros(4)
+        self._call("dsmc_get_totals", _ptr(t))
+        return t
+
+    def collide_totals(self):
+        t = np.zeros(8)
+        self._call("dsmc_get_collide_totals", _ptr(t))
+        return t[:4], t[4:]
+
+    def cells(self):
+        n = self.ncells
+        out = {k: np.zeros(n) for k in ("coeff", "vr_max", "rest")}
+        cnt = np.zeros(n, np.int32)
+        self._call("dsmc_get_cells", _ptr(out["coeff"]), _ptr(out["vr_max"]), _ptr(out["rest"]), _ptr(cnt))
+        out["count"] = cnt
+        out["active"] = out["coeff"] > 0
+        return out
+
+    def set_cells(self, vr_max=None, rest=None):
+        a = None if vr_max is None else np.ascontiguousarray(vr_max, dtype=np.float64)
+        b = None if rest is None else np.ascontiguousarray(rest, dtype=np.float64)
+        self._call("dsmc_set_cells", None if a is None else _ptr(a), None if b is None else _ptr(b))
+
+    def collide_stats(self):
+        p = np.zeros(self.ncells, np.int32)
+        a = np.zeros(self.ncells, np.int32)
+        self._call("dsmc_get_collide_stats", _ptr(p), _ptr(a))
+        return p, a
+
+    def last_perm(self):
+        p = np.zeros(self.n, np.int32)
+        self._call("dsmc_get_last_perm", _ptr(p))
+        return p
+
+    def last_wall_hits(self):
+        nh = np.zeros(self.n, np.int32)
+        lv = np.zeros(self.n, np.int32)
+        self._call("dsmc_get_last_wall_hits", _ptr(nh), _ptr(lv))
+        return nh, lv
+
+    def kernel_times(self):
+        n = ctypes.c_int(32)
+        names = ctypes.create_string_buffer(4096)
+        ms = np.zeros(32)
+        self._call("dsmc_get_kernel_times", ctypes.byref(n), names, 4096, _ptr(ms))
+        keys = [k for k in names.value.decode().split(";") if k]
+        return dict(zip(keys, ms[: n.value].tolist()))
+
+    def save_state_file(self, path):
+        self._call("dsmc_save_state_file", os.fsencode(path))
+
+    def load_state_file(self, path):
+        self._call("dsmc_load_state_file", os.fsencode(path))
+
+
+def gpu_available() -> bool:
+    """True when a HIP device is visible (checked without initialising HIP in
+    this process beyond the device count)."""
+    try:
+        import torch  # noqa: F401  (plumbing only: device discovery)
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False

@@ -42,6 +42,7 @@ def lib():
         _lib.or_voxel_index_halo.restype = L
         _lib.or_cell_index_from_position.restype = L
         _lib.or_geom_porosity.restype = ctypes.c_float
+        _lib.or_system_wall_log.restype = L
     return _lib
 
 
@@ -290,6 +291,23 @@ class OracleSystem:
         m = np.zeros(self.n)
         self.lib.or_system_last_margin(P(self.h), _ptr(m))
         return m
+
+    def move_dt(self, dt):
+        self.lib.or_system_move_dt(P(self.h), D(dt))
+
+    def enable_wall_log(self, capacity=1_000_000):
+        self.lib.or_system_wall_log_enable(P(self.h), L(capacity))
+
+    def wall_log(self, clear=True):
+        n = self.lib.or_system_wall_log(P(self.h), None, None, None, None)
+        vox = np.zeros(max(n, 1), np.int64)
+        part = np.zeros(max(n, 1), np.int64)
+        vin = np.zeros(max(3 * n, 1))
+        vout = np.zeros(max(3 * n, 1))
+        self.lib.or_system_wall_log(P(self.h), _ptr(vox), _ptr(part), _ptr(vin), _ptr(vout))
+        if clear:
+            self.lib.or_system_wall_log_clear(P(self.h))
+        return vox[:n], part[:n], vin[:3 * n].reshape(n, 3), vout[:3 * n].reshape(n, 3)
 
     def sample(self):
         self.lib.or_system_sample(P(self.h))

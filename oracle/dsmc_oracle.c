@@ -426,6 +426,10 @@ struct or_system {
     long nsamples;
     long cur_index;         /* particle index of the move in progress  */
     int cur_hit;
+    /* wall-hit log */
+    long log_cap, log_n;
+    int64_t *log_voxel, *log_part;
+    double *log_vin, *log_vout;
 };
 
 static inline long halo_index(const or_system *s, int i, int j, int k) {
@@ -606,6 +610,12 @@ static void find_position(or_system *s, long index, int tag_step) {
 static void wall_collide(or_system *s, double *v, long cvi_halo, long index, int hit) {
     int ok;
     long g = global_from_halo(s, cvi_halo, &ok);
+    long li = s->log_n;
+    if (li < s->log_cap) {
+        s->log_voxel[li] = g;
+        s->log_part[li] = index;
+        memcpy(s->log_vin + 3 * li, v, 3 * sizeof(double));
+    }
     const float *n = s->nrm + 3 * g, *a = s->tg1 + 3 * g, *b = s->tg2 + 3 * g;
     if (s->p.wall_model == OR_WALL_SPECULAR) {                   /* colliderspecular.cpp:8-32 */
         float n_x = n[0], n_y = n[1], n_z = n[2];
@@ -640,6 +650,7 @@ static void wall_collide(or_system *s, double *v, long cvi_halo, long index, int
         v[1] = v_normal * n_y + v_tangent1 * t1_y + v_tangent2 * t2_y;
         v[2] = v_normal * n_z + v_tangent1 * t1_z + v_tangent2 * t2_z;
     }
+    if (li < s->log_cap) { memcpy(s->log_vout + 3 * li, v, 3 * sizeof(double)); s->log_n++; }
     s->cnt.wall_collisions++;
     s->nhits[index]++;
     s->lasthit[index] = (int32_t)g;
@@ -875,6 +886,7 @@ void or_system_destroy(or_system *s) {
     free(s->mol_index_in_cell); free(s->mol_cell_index);
     free(s->start); free(s->perm); free(s->nhits); free(s->lasthit); free(s->margin);
     free(s->cell_pairs); free(s->cell_acc); free(s->mom);
+    free(s->log_voxel); free(s->log_part); free(s->log_vin); free(s->log_vout);
     free(s);
 }
 
@@ -1102,6 +1114,31 @@ int or_system_step(or_system *s, int phase_mask) {        /* system.cpp:22-47 */
 }
 
 void or_system_counters(const or_system *s, or_counters_t *c) { *c = s->cnt; }
+
+void or_system_move_dt(or_system *s, double dt) {
+    for (long i = 0; i < s->n; i++) { s->nhits[i] = 0; s->lasthit[i] = -1; s->margin[i] = 1e300; }
+    for (long i = 0; i < s->n; i++) { s->cur_index = i; s->cur_hit = 0; move_molecule(s, i, dt); }
+    for (long i = 0; i < s->n; i++) periodic(s, s->r + 3 * i);
+}
+
+void or_system_wall_log_enable(or_system *s, long capacity) {
+    s->log_cap = capacity;
+    s->log_n = 0;
+    s->log_voxel = (int64_t *)realloc(s->log_voxel, sizeof(int64_t) * (capacity ? capacity : 1));
+    s->log_part = (int64_t *)realloc(s->log_part, sizeof(int64_t) * (capacity ? capacity : 1));
+    s->log_vin = (double *)realloc(s->log_vin, 3 * sizeof(double) * (capacity ? capacity : 1));
+    s->log_vout = (double *)realloc(s->log_vout, 3 * sizeof(double) * (capacity ? capacity : 1));
+}
+
+long or_system_wall_log(const or_system *s, int64_t *voxel, int64_t *particle, double *vin, double *vout) {
+    if (voxel) memcpy(voxel, s->log_voxel, sizeof(int64_t) * s->log_n);
+    if (particle) memcpy(particle, s->log_part, sizeof(int64_t) * s->log_n);
+    if (vin) memcpy(vin, s->log_vin, 3 * sizeof(double) * s->log_n);
+    if (vout) memcpy(vout, s->log_vout, 3 * sizeof(double) * s->log_n);
+    return s->log_n;
+}
+
+void or_system_wall_log_clear(or_system *s) { s->log_n = 0; }
 
 int or_system_num_cells(const or_system *s) { return s->ncells; }
 

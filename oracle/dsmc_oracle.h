@@ -167,6 +167,15 @@ void   or_system_reset_moments(or_system *s);
 /* FP64 totals: [0..2] momentum sum, [3] sum |v|^2. */
 void   or_system_totals(const or_system *s, double *tot4);
 
+/* Move every particle with a caller-chosen dt (no accelerate, no step
+ * increment) and wrap: the body of System::move() (system.cpp:107-120). */
+void   or_system_move_dt(or_system *s, double dt);
+/* Wall-hit log (every surface-collider call): enable with a capacity, read
+ * voxel (global linear id), particle, v before and after. */
+void   or_system_wall_log_enable(or_system *s, long capacity);
+long   or_system_wall_log(const or_system *s, int64_t *voxel, int64_t *particle, double *vin, double *vout);
+void   or_system_wall_log_clear(or_system *s);
+
 /* Stand-alone pieces used by unit tests. */
 double or_time_until_collision(const or_system *s, const double *r, const double *v, long voxel_halo_index);
 long   or_voxel_index_halo(const or_system *s, const double *r);

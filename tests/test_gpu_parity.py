@@ -1,0 +1,231 @@
+"""GPU (libdsmc.so through the C ABI) against the FP64 CPU oracle.
+
+Both run the same counter-based draws (Philox4x32-10 keyed by step, particle
+or cell, and sub-index), so results compare per particle and per candidate
+pair.  Tolerances (FP32 device state against the FP64 oracle):
+  * cell indices, sort permutation, wall-hit voxel ids, pair selections,
+    pair counts: bit-exact;
+  * positions: |dr| <= 1e-6 * max(|r|, h) (relative, floored at one voxel edge h);
+  * velocities: |dv| <= 1e-6 * |v| per particle;
+  * decisions that the FP64 oracle takes closer than 1e-5 voxel to a voxel
+    face (FP32 cannot resolve them) are excluded and counted; at most
+    1e-3 of the particles may be excluded.  Collide: per-cell accepted
+    counts and velocities are exact/1e-6 except in cells where an accept
+    test lies within FP32 rounding of its threshold (at most 2 cells).
+"""
+import numpy as np
+import pytest
+
+from tests import dsmc_cases as K
+
+pytestmark = pytest.mark.gpu
+
+AMBIG = 1e-5
+
+
+def _init_pair(gpu, n_particles=200_000, wall="thermal", seed=1, gravity=None, w=None, dt_tau=0.2):
+    w = w or K.world()
+    ini = K.settings(w, n_particles, seed=seed, wall=wall, gravity=gravity, dt_tau=dt_tau)
+    g = K.gpu_system(ini, w)
+    g.initialize()
+    st = g.state()
+    o = K.oracle_system(ini, w, state=st)
+    return ini, w, g, o, st
+
+
+def test_seeding_matches_oracle(gpu):
+    w = K.world()
+    ini = K.settings(w, 200_000)
+    g = K.gpu_system(ini, w)
+    n = g.initialize()
+    o = K.oracle_system(ini, w)
+    assert n == o.n
+    a, b = g.state(), o.state()
+    assert np.array_equal(a[:, :3], b[:, :3])           # integer voxel pick + exact offsets
+    assert K.rel_vec_err(a[:, 3:], b[:, 3:]).max() < 1e-6
+    # stable cell order
+    c = K.cell_of_positions(a, ini)
+    assert np.all(np.diff(c) >= 0)
+    assert g.counters()["errors"] == 0
+
+
+@pytest.mark.parametrize("wall,mult", [("thermal", 1), ("specular", 1), ("thermal", 10)])
+def test_move_and_sort_match_oracle(gpu, wall, mult):
+    ini, w, g, o, st0 = _init_pair(gpu, wall=wall, gravity=0.05, dt_tau=0.2 * mult)
+    g.set_option(K.dsmc.OPT_RECORD, 1)
+    g.set_option(K.dsmc.OPT_PHASES, 1)          # move + sort, no collide
+    g.step(1)
+    o.step(1 | 2)
+    margin = o.last_margin()
+    amb = margin < AMBIG
+    assert amb.mean() < 1e-3
+    # wall hits (per index before the step)
+    nh_g, lv_g = g.last_wall_hits()
+    nh_o, lv_o = o.last_wall()
+    ok = ~amb
+    assert np.array_equal(nh_g[ok], nh_o[ok])
+    assert np.array_equal(lv_g[ok], lv_o[ok])
+    assert nh_o.sum() > 100
+    # permutation: identical when no decision is ambiguous, else identical
+    # restricted to unambiguous particles (same cell, same relative order)
+    pg, po = g.last_perm(), o.last_perm()
+    if not amb.any():
+        assert np.array_equal(pg, po)
+    else:
+        keep_g = ~amb[pg]
+        keep_o = ~amb[po]
+        assert np.array_equal(pg[keep_g], po[keep_o])
+    # states
+    a, b = g.state(), o.state()
+    if not amb.any():
+        h = float(ini["Lx"]) / 32
+        dr = np.abs(a[:, :3] - b[:, :3]).max(axis=1)
+        assert np.all(dr <= 1e-6 * np.maximum(np.abs(b[:, :3]).max(axis=1), h))
+        assert K.rel_vec_err(a[:, 3:], b[:, 3:]).max() <= 1e-6
+        assert np.array_equal(K.cell_of_positions(a, ini), K.cell_of_positions(b, ini))
+    cg, co = g.counters(), o.counters()
+    if not amb.any():
+        assert cg["wall_collisions"] == co["wall_collisions"]
+        assert cg["count_periodic"] == co["count_periodic"]
+    assert cg["errors"] == 0
+
+
+def test_collide_matches_oracle(gpu):
+    ini, w, g, o, st0 = _init_pair(gpu)
+    cells = o.cells()
+    rng = np.random.default_rng(5)
+    rest = rng.uniform(-0.5, 0.5, len(cells["rest"]))
+    g.set_cells(rest=rest)
+    o.set_cell_state(rest=rest)
+    g.set_option(K.dsmc.OPT_PHASES, 4)          # collide only
+    g.step(1)
+    o.step(4)
+    pg, ag = g.collide_stats()
+    po, ao = o.collide_stats()
+    act = cells["active"]
+    assert np.array_equal(pg[act], po[act])
+    bad = np.nonzero(ag[act] != ao[act])[0]
+    assert len(bad) <= 2
+    gc, oc = g.cells(), o.cells()
+    assert np.array_equal(gc["rest"][act], oc["rest"][act])
+    a, b = g.state(), o.state()
+    assert np.array_equal(a[:, :3], b[:, :3])
+    cell = K.cell_of_positions(b, ini)
+    bad_cells = np.nonzero(act)[0][bad]
+    keep = ~np.isin(cell, bad_cells)
+    assert K.rel_vec_err(a[keep, 3:], b[keep, 3:]).max() <= 1e-6
+    good = np.setdiff1d(np.nonzero(act)[0], bad_cells)
+    assert np.allclose(gc["vr_max"][good], oc["vr_max"][good], rtol=1e-6)
+    assert ag.sum() > 10000
+
+
+def test_full_steps_statistics(gpu):
+    """50 full steps: N conserved, collision rate and wall-hit rate agree with
+    the oracle's own 50 steps from the same state within 3 sigma."""
+    ini, w, g, o, st0 = _init_pair(gpu, gravity=0.05)
+    g.step(50)
+    for _ in range(50):
+        o.step()
+    cg, co = g.counters(), o.counters()
+    assert cg["num_molecules"] == len(st0)
+    for k in ("collisions", "wall_collisions"):
+        x, y = cg[k], co[k]
+        assert abs(x - y) < 3 * np.sqrt(x + y) + 10, (k, x, y)
+    assert cg["errors"] == 0
+    a = g.state()
+    c = K.cell_of_positions(a, ini)
+    assert np.all(np.diff(c) >= 0)
+
+
+def test_collision_conservation(gpu):
+    ini, w, g, o, st0 = _init_pair(gpu)
+    g.set_option(K.dsmc.OPT_TOTALS, 1)
+    g.step(3)
+    before, after = g.collide_totals()
+    n = len(st0)
+    # momentum and energy change only by FP32 rounding of the collisions
+    assert np.all(np.abs(after[:3] - before[:3]) < 1e-5 * np.sqrt(before[3]))
+    assert abs(after[3] - before[3]) < 2e-6 * before[3]
+    assert g.counters()["collisions"] > 0.05 * n
+
+
+def test_specular_preserves_speed(gpu):
+    ini, w, g, o, st0 = _init_pair(gpu, wall="specular", dt_tau=2.0)
+    g.set_option(K.dsmc.OPT_PHASES, 1)
+    g.set_option(K.dsmc.OPT_RECORD, 1)
+    g.step(1)
+    perm = g.last_perm()
+    a = g.state()
+    s0 = np.linalg.norm(st0[perm, 3:], axis=1)
+    s1 = np.linalg.norm(a[:, 3:], axis=1)
+    nh, _ = g.last_wall_hits()
+    assert nh.sum() > 1000
+    assert np.allclose(s1, s0, rtol=2e-6)
+
+
+def test_graph_replay_is_bit_identical(gpu):
+    w = K.world()
+    ini = K.settings(w, 100_000, gravity=0.05)
+    a = K.gpu_system(ini, w)
+    a.initialize()
+    b = K.gpu_system(ini, w)
+    b.initialize()
+    b.set_option(K.dsmc.OPT_USE_GRAPH, 1)
+    a.set_sampling(1)
+    b.set_sampling(1)
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(a.state(), b.state())
+    ma, _ = a.moments()
+    mb, _ = b.moments()
+    assert np.array_equal(ma, mb)
+    assert a.counters()["collisions"] == b.counters()["collisions"]
+
+
+def test_moments_match_bruteforce(gpu):
+    ini, w, g, o, st0 = _init_pair(gpu)
+    g.set_sampling(1)
+    g.step(2)
+    m, ns = g.moments()
+    assert ns == 2
+    a = g.state()
+    c = K.cell_of_positions(a, ini)
+    v = a[:, 3:]
+    last = np.zeros((g.ncells, 7))
+    np.add.at(last[:, 0], c, 1)
+    for j in range(3):
+        np.add.at(last[:, 1 + j], c, v[:, j])
+        np.add.at(last[:, 4 + j], c, v[:, j] ** 2)
+    g.reset_moments()
+    g.step(1)
+    m1, ns1 = g.moments()
+    a = g.state()
+    c = K.cell_of_positions(a, ini)
+    v = a[:, 3:]
+    want = np.zeros((g.ncells, 7))
+    np.add.at(want[:, 0], c, 1)
+    for j in range(3):
+        np.add.at(want[:, 1 + j], c, v[:, j])
+        np.add.at(want[:, 4 + j], c, v[:, j] ** 2)
+    assert ns1 == 1
+    assert np.allclose(m1, want, rtol=1e-12, atol=1e-12)
+
+
+def test_large_system_invariants(gpu):
+    """Precision-form size (64^3, 16^3 cells, 2M particles): sorted order,
+    bijective permutation, conserved count, zero device errors."""
+    w = K.world(64, 6.0, 0.5, 2)
+    ini = K.settings(w, 2_000_000, gravity=0.05)
+    g = K.gpu_system(ini, w)
+    n = g.initialize()
+    g.set_option(K.dsmc.OPT_RECORD, 1)
+    g.step(5)
+    perm = g.last_perm()
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    a = g.state()
+    c = K.cell_of_positions(a, ini)
+    assert np.all(np.diff(c) >= 0)
+    cnt = g.counters()
+    assert cnt["errors"] == 0 and cnt["num_molecules"] == n
+    cells = g.cells()
+    assert np.array_equal(np.bincount(c, minlength=g.ncells), cells["count"])

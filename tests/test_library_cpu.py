@@ -1,0 +1,107 @@
+"""Host-side checks of master_amd/libdsmc.so that need no GPU: the C ABI is
+complete, the settings/world readers accept the reference formats, and the
+library's geometry generator reproduces the reference's worlds."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from master_amd import config as C
+from master_amd import dsmc
+from oracle import binding as B
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "dsmc.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(dsmc_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = dsmc.lib()
+    names = declared_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_no_oracle_in_product():
+    """The product library never links or names the oracle."""
+    for src in os.listdir(os.path.join(ROOT, "master_amd", "csrc")):
+        text = open(os.path.join(ROOT, "master_amd", "csrc", src), errors="ignore").read()
+        assert "oracle" not in text.lower(), src
+    for py in ("dsmc.py", "config.py", "__init__.py"):
+        assert "oracle" not in open(os.path.join(ROOT, "master_amd", py)).read()
+
+
+@pytest.mark.parametrize("n,r,phi,seed", [(32, 4.0, 0.6, 1), (64, 6.0, 0.5, 2)])
+def test_library_packed_spheres_equal_reference_generator(golden, n, r, phi, seed):
+    import hashlib
+    w = dsmc.WorldData.packed_spheres(n, r, phi, seed)
+    h = hashlib.sha256()
+    for a in (w.types, w.normals, w.t1, w.t2):
+        h.update(a.tobytes())
+    name = "g32" if n == 32 else "g64"
+    assert np.array_equal(np.frombuffer(h.digest(), np.uint8), golden[f"{name}_digest"])
+
+
+def test_library_poly_spheres_equal_oracle():
+    w = dsmc.WorldData.poly_spheres((40, 32, 48), 2.0, 5.0, 0.5, 17)
+    o = B.poly_spheres((40, 32, 48), 2.0, 5.0, 0.5, 17)
+    for a, k in ((w.types, "types"), (w.normals, "normals"), (w.t1, "t1"), (w.t2, "t2")):
+        assert np.array_equal(a, o[k])
+    assert w.porosity == pytest.approx(o["porosity"])
+
+
+def test_world_file_round_trip(tmp_path):
+    """dsmc_io.cpp:183-220 layout: the library reads what the reference
+    writer (complexgeometry.cpp:220-318) writes, and writes the same bytes."""
+    w = B.packed_spheres(32, 4.0, 0.6, 1)
+    B.write_world_file(str(tmp_path / "ref"), w)
+    got = dsmc.WorldData.read(str(tmp_path / "ref" / "0000.bin"))
+    assert np.array_equal(got.types, w["types"])
+    assert np.array_equal(got.normals, w["normals"])
+    assert np.array_equal(got.t1, w["t1"]) and np.array_equal(got.t2, w["t2"])
+    assert np.float32(got.porosity) == np.float32(w["porosity"])
+    got.write(str(tmp_path / "mine.bin"))
+    assert open(tmp_path / "mine.bin", "rb").read() == open(tmp_path / "ref" / "0000.bin", "rb").read()
+
+
+def test_settings_from_reference_ini(tmp_path):
+    """settings.cpp:3-44 keys and System::initialize's unit conversion."""
+    w = B.packed_spheres(32, 4.0, 0.6, 1)
+    ini = C.dsmc_settings(32, w["porosity"], 200_000, 0.1, 8, seed=3)
+    B.write_ini(str(tmp_path / "dsmc.ini"), dict(ini, world="world_dir"))
+    s, world = dsmc.settings_from_ini_file(str(tmp_path / "dsmc.ini"))
+    assert world == "world_dir"
+    p = B.params_from_settings(ini, w)
+    assert s.temperature == p.temperature and s.density == p.density and s.dt == p.dt
+    assert s.cells_x == 8 and s.atoms_per_molecule == int(p.atoms_per_molecule) and s.seed == 3
+    s2 = dsmc.settings_from_ini(ini)
+    for f, _ in dsmc.Settings._fields_:
+        assert getattr(s, f) == getattr(s2, f), f
+
+
+def test_invalid_configuration_is_an_error_not_an_exit():
+    w = B.packed_spheres(32, 4.0, 0.6, 1)
+    ini = C.dsmc_settings(32, w["porosity"], 1000, 0.1, 8)
+    ini["cells_x"] = 6                  # 32 % 6 != 0 (system.cpp:221-239)
+    with pytest.raises(dsmc.DsmcError):
+        dsmc.System(ini, w)
+
+
+def test_config_presets_reproduce_survey_numbers():
+    w = B.packed_spheres(32, 4.0, 0.6, 1)
+    ini = C.dsmc_settings(32, w["porosity"], 200_000, 0.1, 8)
+    assert float(ini["density"]) == pytest.approx(5.3768e26, rel=1e-4)
+    assert float(ini["dt"]) == pytest.approx(1.5624e-3, rel=1e-3)
+    assert abs(int(ini["atoms_per_molecule"]) - 1500) <= 1
+    ini["atoms_per_molecule"] = 1500
+    p = B.params_from_settings(ini, w)
+    o = B.OracleSystem(p, w)
+    # N = (long)(n V phi / Fn) (system.cpp:313-323): ~200 142 for Fn = 1500
+    assert abs(o.seed() - 200142) <= 2
