@@ -34,6 +34,8 @@ struct Member {
     float dt;            // time step
     float gdt;           // gravity * dt
     int flow_dir;        // -1: no body force
+    float acc[3];        // gravity * dt on the flow axis, exactly 0 on the others
+    float inv_b[3];      // 1 / voxels per cell, rounded up
     float s_wall;        // sqrt(T_wall / m)
     float s_gas;         // sqrt(T / m)
     int wall_model;
@@ -78,8 +80,12 @@ __device__ __forceinline__ int wrapi(int i, int n) {
 
 // Global cell id of a (wrapped) voxel, cell_index_from_ijk ordering
 // (system.h:130-132): cx*cy*cz + cy*cz + cz.
+// floor(i / b) for 0 <= i < 2^20, b <= 8: i * (1/b rounded up) never reaches
+// the next integer, so the float product truncates to the exact quotient.
+__device__ __forceinline__ int div_b(int i, float inv_b) { return (int)((float)i * inv_b); }
+
 __device__ __forceinline__ uint32_t cell_of(const Member &M, int ix, int iy, int iz) {
-    int cx = ix / M.b[0], cy = iy / M.b[1], cz = iz / M.b[2];
+    int cx = div_b(ix, M.inv_b[0]), cy = div_b(iy, M.inv_b[1]), cz = div_b(iz, M.inv_b[2]);
     return M.cell0 + (uint32_t)((cx * M.cpa[1] + cy) * M.cpa[2] + cz);
 }
 
@@ -123,27 +129,51 @@ __device__ __forceinline__ void advance1(int &i, float &o, float w, float tau) {
 // boundary voxel's lower corner) to reach a face of the unit cube, checking
 // the six faces in the reference's order (x-, x+, y-, y+, z-, z+), each hit
 // point inside its closed face square, minimum positive time wins.
-__device__ __forceinline__ float face_time(float dx, float dy, float dz, float wx, float wy,
-                                           float wz, bool &no_face) {
+// The particle is in voxel i (offset o) and the boundary voxel's index on the
+// same axis is b; with d = i - b the ray crosses the voxel's low plane after
+// ((-d) - o) / w and its high plane after ((1 - d) - o) / w.  Taking the
+// integer difference first keeps the plane distance exact when the particle
+// sits next to the plane (d = -1 or +1), where the face decisions are made.
+__device__ __forceinline__ float face_time(int dx, int dy, int dz, float ox, float oy, float oz,
+                                           float wx, float wy, float wz, bool &no_face, int &face) {
     float best = 1e9f;
     float t;
-#define DSMC_FACE(T, A, WA, B, WB)                                                         \
+    face = -1;
+#define DSMC_FACE(F, T, DA, OA, WA, DB, OB, WB)                                            \
     t = (T);                                                                               \
     {                                                                                      \
-        float pa = (A) + (WA) * t, pb = (B) + (WB) * t;                                    \
+        float pa = (OA + (WA) * t) + (float)(DA), pb = (OB + (WB) * t) + (float)(DB);      \
         bool in = pa >= 0.0f && pa <= 1.0f && pb >= 0.0f && pb <= 1.0f;                    \
-        if (t > 0.0f && t < best && in && !isnan(t)) best = t;                             \
+        if (t > 0.0f && t < best && in && !isnan(t)) { best = t; face = (F); }             \
     }
-    DSMC_FACE((0.0f - dx) / wx, dy, wy, dz, wz)
-    DSMC_FACE((1.0f - dx) / wx, dy, wy, dz, wz)
-    DSMC_FACE((0.0f - dy) / wy, dx, wx, dz, wz)
-    DSMC_FACE((1.0f - dy) / wy, dx, wx, dz, wz)
-    DSMC_FACE((0.0f - dz) / wz, dx, wx, dy, wy)
-    DSMC_FACE((1.0f - dz) / wz, dx, wx, dy, wy)
+    DSMC_FACE(0, ((float)(-dx) - ox) / wx, dy, oy, wy, dz, oz, wz)
+    DSMC_FACE(1, ((float)(1 - dx) - ox) / wx, dy, oy, wy, dz, oz, wz)
+    DSMC_FACE(2, ((float)(-dy) - oy) / wy, dx, ox, wx, dz, oz, wz)
+    DSMC_FACE(3, ((float)(1 - dy) - oy) / wy, dx, ox, wx, dz, oz, wz)
+    DSMC_FACE(4, ((float)(-dz) - oz) / wz, dx, ox, wx, dy, oy, wy)
+    DSMC_FACE(5, ((float)(1 - dz) - oz) / wz, dx, ox, wx, dy, oy, wy)
 #undef DSMC_FACE
     no_face = (best == 1e9f);
-    if (best < 1e-5f) return 0.0f;
+    if (best < 1e-5f) { face = -1; return 0.0f; }
     return best - 1e-7f;
+}
+
+// Moving to face_time - 1e-7 leaves the particle, in the reference's FP64,
+// outside the boundary voxel at |w_a| * 1e-7 voxels from the entry plane
+// (plane - v_a 1e-7).  That distance is below one FP32 ulp of the offset for
+// slow normal velocities, so the entry axis is set from it directly: the
+// particle stays in the neighbour across the entry face, a positive distance
+// away, so a later face solve sees the plane ahead of it.
+__device__ __forceinline__ void settle_outside(int face, int b, float w, int &i, float &o) {
+    float d = fabsf(w) * 1e-7f;
+    if ((face & 1) == 0) {              // through the low plane: voxel b-1, offset 1 - d
+        i = b - 1;
+        o = 1.0f - d;
+        if (!(o < 1.0f)) o = 0x1.fffffep-1f;
+    } else {                            // through the high plane: voxel b+1, offset d
+        i = b + 1;
+        o = d > 0.0f ? d : 0x1p-126f;
+    }
 }
 
 // Re-seed uniformly in the pore space (System::find_position,
@@ -208,9 +238,11 @@ __device__ __forceinline__ int32_t move_particle(const Member &M, const VoxelVie
                                                  float &ox, float &oy, float &oz, float &vx,
                                                  float &vy, float &vz, MoveCounters &cnt,
                                                  uint32_t &nhit) {
-    if (M.flow_dir == 0) vx = vx + M.gdt;
-    else if (M.flow_dir == 1) vy = vy + M.gdt;
-    else if (M.flow_dir == 2) vz = vz + M.gdt;
+    // System::accelerate: v[flow_dir] += gravity*dt; adding an exact 0 to the
+    // other components leaves them unchanged
+    vx = vx + M.acc[0];
+    vy = vy + M.acc[1];
+    vz = vz + M.acc[2];
     float dt = M.dt;
     int32_t last = -1;
     uint32_t hit = 0;
@@ -218,10 +250,22 @@ __device__ __forceinline__ int32_t move_particle(const Member &M, const VoxelVie
     for (int depth = 0;; depth++) {
         float tau = dt;
         if (V.at(ix, iy, iz) != V_EMPTY) { reseeded = true; break; }
+        // Every position of this flight is start + w*tau: the reference's
+        // bisection (tau /= 2; r -= v*tau) and its back-off before the face
+        // solve (r -= v*tau) return to the flight's start point in exact
+        // arithmetic, so positions are recomputed from the start instead of
+        // accumulating rounding.
+        const int sx = ix, sy = iy, sz = iz;
+        const float so_x = ox, so_y = oy, so_z = oz;
         float wx = vx * M.inv_h[0], wy = vy * M.inv_h[1], wz = vz * M.inv_h[2];
-        advance1(ix, ox, wx, tau);
-        advance1(iy, oy, wy, tau);
-        advance1(iz, oz, wz, tau);
+#define DSMC_FLY(T)                                                        \
+        do {                                                               \
+            ix = sx; iy = sy; iz = sz; ox = so_x; oy = so_y; oz = so_z;    \
+            advance1(ix, ox, wx, (T));                                     \
+            advance1(iy, oy, wy, (T));                                     \
+            advance1(iz, oz, wz, (T));                                     \
+        } while (0)
+        DSMC_FLY(tau);
         uint32_t t = V.at(ix, iy, iz);
         if (t >= V_WALL) {
             int count = 0;
@@ -230,9 +274,7 @@ __device__ __forceinline__ int32_t move_particle(const Member &M, const VoxelVie
                 if (++count > 100) { bad = true; break; }
                 if (t == V_WALL) {
                     tau = tau / 2.0f;
-                    advance1(ix, ox, wx, -tau);
-                    advance1(iy, oy, wy, -tau);
-                    advance1(iz, oz, wz, -tau);
+                    DSMC_FLY(tau);
                     t = V.at(ix, iy, iz);
                 } else {
                     dt = dt - tau;
@@ -250,18 +292,19 @@ __device__ __forceinline__ int32_t move_particle(const Member &M, const VoxelVie
             while (t == V_BOUNDARY) {
                 if (++count > 100) { bad = true; break; }
                 bx = ix; by = iy; bz = iz;
-                advance1(ix, ox, wx, -tau);
-                advance1(iy, oy, wy, -tau);
-                advance1(iz, oz, wz, -tau);
                 bool no_face;
-                tau = face_time((float)(ix - bx) + ox, (float)(iy - by) + oy, (float)(iz - bz) + oz,
-                                wx, wy, wz, no_face);
+                int face;
+                tau = face_time(sx - bx, sy - by, sz - bz, so_x, so_y, so_z, wx, wy, wz, no_face, face);
                 if (no_face) { cnt.errors++; bad = true; break; }
-                advance1(ix, ox, wx, tau);
-                advance1(iy, oy, wy, tau);
-                advance1(iz, oz, wz, tau);
+                DSMC_FLY(tau);
+                if (face >= 0) {
+                    if (face < 2) settle_outside(face, bx, wx, ix, ox);
+                    else if (face < 4) settle_outside(face, by, wy, iy, oy);
+                    else settle_outside(face, bz, wz, iz, oz);
+                }
                 t = V.at(ix, iy, iz);
             }
+#undef DSMC_FLY
             if (bad) { reseeded = true; break; }
             dt = dt - tau;
             int gx = wrapi(bx, M.n[0]), gy = wrapi(by, M.n[1]), gz = wrapi(bz, M.n[2]);

@@ -168,6 +168,13 @@ dsmc_status build_member(dsmc_handle *h, const dsmc_settings &s, const dsmc_worl
     // System::accelerate skips when flow_direction < 0 or gravity == 0 (133-134)
     d.flow_dir = (s.flow_direction < 0 || s.flow_direction > 2 || s.gravity == 0) ? -1 : s.flow_direction;
     d.gdt = (float)(s.gravity * s.dt);
+    for (int a = 0; a < 3; a++) {
+        d.acc[a] = (a == d.flow_dir) ? d.gdt : 0.0f;
+        // smallest float >= 1/b: exact truncating division for small indices
+        float inv = 1.0f / (float)b[a];
+        if ((double)inv * b[a] < 1.0) inv = std::nextafter(inv, 2.0f);
+        d.inv_b[a] = inv;
+    }
     d.s_wall = (float)std::sqrt(s.wall_temperature / s.mass);
     d.s_gas = (float)std::sqrt(s.temperature / s.mass);
     d.wall_model = s.wall_model;
@@ -408,6 +415,9 @@ BucketArgs bucket_args(dsmc_handle *h) {
     b.bstart = h->d_bstart;
     b.bfill = h->d_bfill;
     b.bucket = h->d_bucket;
+    b.start = h->d_start[h->scur];
+    b.leave = h->d_leave;
+    b.ncells = h->ncells;
     return b;
 }
 
@@ -483,8 +493,8 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
         HIPCHK(h, launch_scan(scan_args(h), s));
     }
     {
-        Scope sc(h, "k_bucket");
-        HIPCHK(h, launch_bucket(bucket_args(h), s));
+        Scope sc(h, "k_bucket_cells");
+        HIPCHK(h, launch_bucket_cells(bucket_args(h), s));
     }
     {
         Scope sc(h, "k_gather_collide");

@@ -49,7 +49,9 @@ struct ScanArgs {
 
 struct BucketArgs {
     const uint32_t *mv_idx, *mv_cell, *mv_count, *bstart;
+    const uint32_t *start, *leave;      // per-cell mover lists (k_bucket_cells)
     uint32_t *bfill, *bucket;
+    uint32_t ncells;
 };
 
 struct GatherArgs {
@@ -86,6 +88,7 @@ hipError_t launch_move(const MoveArgs &a, uint32_t ncells, hipStream_t s);
 hipError_t launch_classify_all(const ClassifyArgs &a, hipStream_t s);
 hipError_t launch_scan(const ScanArgs &a, hipStream_t s);
 hipError_t launch_bucket(const BucketArgs &a, hipStream_t s);
+hipError_t launch_bucket_cells(const BucketArgs &a, hipStream_t s);
 hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);

@@ -68,15 +68,30 @@ __global__ __launch_bounds__(64) void k_move(MoveArgs a) {
     const uint32_t step = *a.step;
     MoveCounters cnt = {0, 0, 0, {0, 0, 0}};
     uint32_t leavers = 0;
+    // software pipeline: the next chunk's loads are in flight while this
+    // chunk is moved
+    uint32_t p = s0 + lane;
+    float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
+    uint32_t vp = 0;
+    if (p < s1) {
+        ox = a.ox[p]; oy = a.oy[p]; oz = a.oz[p];
+        vx = a.vx[p]; vy = a.vy[p]; vz = a.vz[p];
+        vp = a.vox[p];
+    }
     for (uint32_t base = s0; base < s1; base += WAVE) {
-        const uint32_t p = base + lane;
+        const uint32_t pn = p + WAVE;
+        float nox = 0, noy = 0, noz = 0, nvx = 0, nvy = 0, nvz = 0;
+        uint32_t nvp = 0;
+        if (pn < s1) {
+            nox = a.ox[pn]; noy = a.oy[pn]; noz = a.oz[pn];
+            nvx = a.vx[pn]; nvy = a.vy[pn]; nvz = a.vz[pn];
+            nvp = a.vox[pn];
+        }
         const bool valid = p < s1;
         uint32_t newc = c;
         if (valid) {
-            float ox = a.ox[p], oy = a.oy[p], oz = a.oz[p];
-            float vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
             int ix, iy, iz;
-            unpack_vox(a.vox[p], ix, iy, iz);
+            unpack_vox(vp, ix, iy, iz);
             uint32_t nhit = 0;
             int32_t last = move_particle(M, V, frames, step, p, ix, iy, iz, ox, oy, oz, vx, vy, vz,
                                          cnt, nhit);
@@ -89,22 +104,19 @@ __global__ __launch_bounds__(64) void k_move(MoveArgs a) {
                 a.rec_last[p] = last;
             }
         }
+        // movers are listed in the cell's own index range, leavers first:
+        // no global counter, no returning atomic
         const bool mover = valid && newc != c;
         const uint64_t mm = __ballot(mover);
-        if (mm) {
-            const uint32_t nm = (uint32_t)__popcll(mm);
-            uint32_t mbase = 0;
-            const uint32_t first = (uint32_t)__ffsll((long long)mm) - 1u;
-            if (lane == first) mbase = atomicAdd(a.mv_count, nm);
-            mbase = __shfl(mbase, (int)first, WAVE);
-            if (mover) {
-                const uint32_t slot = mbase + mbcnt(mm);
-                a.mv_idx[slot] = p;
-                a.mv_cell[slot] = newc;
-                atomicAdd(&a.arrive[newc], 1u);
-            }
-            leavers += nm;
+        if (mover) {
+            const uint32_t slot = s0 + leavers + mbcnt(mm);
+            a.mv_idx[slot] = p;
+            a.mv_cell[slot] = newc;
+            atomicAdd(&a.arrive[newc], 1u);
         }
+        leavers += (uint32_t)__popcll(mm);
+        p = pn;
+        ox = nox; oy = noy; oz = noz; vx = nvx; vy = nvy; vz = nvz; vp = nvp;
     }
     // wave-uniform already; reduce the per-lane counters
     uint32_t wh = wave_sum(cnt.wall_hits), rs = wave_sum(cnt.reseeds), er = wave_sum(cnt.errors);
@@ -229,12 +241,26 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(ScanArgs a) {
     }
 }
 
+// Flat mover list (all particles of an unsorted buffer).
 __global__ __launch_bounds__(256) void k_bucket(BucketArgs a) {
     const uint32_t n = *a.mv_count;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t c = a.mv_cell[i];
         uint32_t slot = atomicAdd(&a.bfill[c], 1u);
         a.bucket[a.bstart[c] + slot] = a.mv_idx[i];
+    }
+}
+
+// Movers listed per source cell (k_move): cell c's leavers occupy
+// mv_idx/mv_cell[start[c] .. start[c] + leave[c]).
+__global__ __launch_bounds__(256) void k_bucket_cells(BucketArgs a) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < a.ncells; c += gridDim.x * blockDim.x) {
+        const uint32_t s0 = a.start[c], nl = a.leave[c];
+        for (uint32_t k = 0; k < nl; k++) {
+            uint32_t d = a.mv_cell[s0 + k];
+            uint32_t slot = atomicAdd(&a.bfill[d], 1u);
+            a.bucket[a.bstart[d] + slot] = a.mv_idx[s0 + k];
+        }
     }
 }
 
@@ -352,15 +378,30 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
     const uint32_t step = *a.step;
     const Member M = a.members[a.cell_member[c]];
 
-    // ---- stayers: stable compaction of the old range by wave ballot
+    // ---- stayers: stable compaction of the old range by wave ballot; all
+    // seven arrays of a chunk are loaded together and the next chunk is in
+    // flight while this one is placed
     uint32_t run = 0;
+    uint32_t p = os + lane;
+    float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
+    uint32_t vp = 0;
+    if (p < oe) {
+        vp = a.vox[p];
+        ox = a.ox[p]; oy = a.oy[p]; oz = a.oz[p];
+        vx = a.vx[p]; vy = a.vy[p]; vz = a.vz[p];
+    }
     for (uint32_t base = os; base < oe; base += WAVE) {
-        const uint32_t p = base + lane;
+        const uint32_t pn = p + WAVE;
+        float nox = 0, noy = 0, noz = 0, nvx = 0, nvy = 0, nvz = 0;
+        uint32_t nvp = 0;
+        if (pn < oe) {
+            nvp = a.vox[pn];
+            nox = a.ox[pn]; noy = a.oy[pn]; noz = a.oz[pn];
+            nvx = a.vx[pn]; nvy = a.vy[pn]; nvz = a.vz[pn];
+        }
         const bool valid = p < oe;
         bool stay = false;
-        uint32_t vp = 0;
         if (valid) {
-            vp = a.vox[p];
             int ix, iy, iz;
             unpack_vox(vp, ix, iy, iz);
             stay = cell_of(M, ix, iy, iz) == c;
@@ -369,12 +410,14 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
         if (stay) {
             const uint32_t L = n_low + run + mbcnt(sm);
             const uint32_t d = ns + L;
-            a.ox2[d] = a.ox[p]; a.oy2[d] = a.oy[p]; a.oz2[d] = a.oz[p];
+            a.ox2[d] = ox; a.oy2[d] = oy; a.oz2[d] = oz;
             a.vox2[d] = vp;
-            S.x[L] = a.vx[p]; S.y[L] = a.vy[p]; S.z[L] = a.vz[p];
+            S.x[L] = vx; S.y[L] = vy; S.z[L] = vz;
             if (a.rec_perm) a.rec_perm[d] = (int32_t)p;
         }
         run += (uint32_t)__popcll(sm);
+        p = pn;
+        ox = nox; oy = noy; oz = noz; vx = nvx; vy = nvy; vz = nvz; vp = nvp;
     }
     // ---- arrivals
     for (uint32_t base = 0; base < nb; base += WAVE) {
@@ -541,6 +584,15 @@ hipError_t launch_scan(const ScanArgs &a, hipStream_t s) {
 
 hipError_t launch_bucket(const BucketArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_bucket, dim3(2048), dim3(256), 0, s, a);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_bucket_cells(const BucketArgs &a, hipStream_t s) {
+    uint32_t blocks = (a.ncells + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bucket_cells, dim3(blocks), dim3(256), 0, s, a);
     LAUNCH_CHECK();
     return hipSuccess;
 }

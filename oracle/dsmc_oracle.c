@@ -505,7 +505,8 @@ static inline int within_square(vec3 p1, vec3 p2, vec3 p3, vec3 p4, vec3 pt) { /
 /* grid.h:82-137.  Returns the reference value; *no_face is set when no face
  * was hit (the reference's unguarded 1e9 sentinel). */
 static double time_until_collision(const or_system *s, const double *r, const double *v,
-                                   long voxel_index, int *no_face, double *edge_margin) {
+                                   long voxel_index, int *no_face, double *edge_margin,
+                                   int *face_out, double *raw_out) {
     vec3 rv = v3(r[0] - 0.0, r[1] - 0.0, r[2] - 0.0);
     vec3 vv = v3(v[0], v[1], v[2]);
     double tuc = OR_SENTINEL;
@@ -539,14 +540,18 @@ static double time_until_collision(const or_system *s, const double *r, const do
     hit[3] = within_square(pl[5], pl[4], pl[6], pl[7], vadd(rv, vscale(vv, tf[3])));
     hit[4] = within_square(pl[4], pl[5], pl[1], pl[0], vadd(rv, vscale(vv, tf[4])));
     hit[5] = within_square(pl[2], pl[3], pl[7], pl[6], vadd(rv, vscale(vv, tf[5])));
+    int winner = -1;
     for (int f = 0; f < 6; f++)
-        if (tf[f] > 0 && tf[f] < tuc && hit[f] && !isnan(tf[f])) tuc = tf[f];
+        if (tf[f] > 0 && tf[f] < tuc && hit[f] && !isnan(tf[f])) { tuc = tf[f]; winner = f; }
+    if (face_out) *face_out = (tuc < 1e-5) ? -1 : winner;
+    if (raw_out) *raw_out = tuc;
     if (edge_margin) {
-        /* distance (voxel units) of each candidate hit point to its face's
-         * edges: near zero means the inside test is FP-ambiguous */
+        /* distance (voxel units) of the hit points of the faces that decide
+         * the result (the winner and every earlier plane crossing) to their
+         * face's edges: near zero means the inside test is FP-ambiguous */
         double m = 1e300;
         for (int f = 0; f < 6; f++) {
-            if (!(tf[f] > 0) || isnan(tf[f]) || isinf(tf[f])) continue;
+            if (!(tf[f] > 0) || isnan(tf[f]) || isinf(tf[f]) || tf[f] > tuc) continue;
             vec3 ph = vadd(rv, vscale(vv, tf[f]));
             double c[3] = {ph.x / vs[0] - i, ph.y / vs[1] - j, ph.z / vs[2] - k};
             for (int a = 0; a < 3; a++) {
@@ -565,7 +570,7 @@ static double time_until_collision(const or_system *s, const double *r, const do
 
 double or_time_until_collision(const or_system *s, const double *r, const double *v, long vi) {
     int nf;
-    return time_until_collision(s, r, v, vi, &nf, NULL);
+    return time_until_collision(s, r, v, vi, &nf, NULL, NULL, NULL);
 }
 
 /* ---------- seeding / re-seeding (system.cpp:384-396) ---------- */
@@ -656,15 +661,33 @@ static void wall_collide(or_system *s, double *v, long cvi_halo, long index, int
     s->lasthit[index] = (int32_t)g;
 }
 
-static inline double face_margin(const or_system *s, const double *r) {
+static inline double face_margin_excl(const or_system *s, const double *r, int skip_axis) {
     double m = 1e300;
     for (int a = 0; a < 3; a++) {
+        if (a == skip_axis) continue;
         double x = r[a] * (a == 0 ? s->nx : a == 1 ? s->ny : s->nz) * s->one_over_length[a];
         double f = x - floor(x);
         double d = f < 1.0 - f ? f : 1.0 - f;
         if (d < m) m = d;
     }
     return m;
+}
+
+static inline double face_margin(const or_system *s, const double *r) {
+    return face_margin_excl(s, r, -1);
+}
+
+/* Distance (voxel units) a particle covers in time |t - 1e-5|: how far a
+ * threshold test on a time (face time < 1e-5, remaining dt > 1e-5) is from
+ * flipping, expressed like the positional margins. */
+static inline double time_margin(const or_system *s, const double *v, double t) {
+    double sp = 0;
+    int gn[3] = {s->nx, s->ny, s->nz};
+    for (int a = 0; a < 3; a++) {
+        double w = fabs(v[a]) * gn[a] * s->one_over_length[a];
+        if (w > sp) sp = w;
+    }
+    return fabs(t - 1e-5) * sp;
 }
 
 static inline void do_move(double *r, const double *v, double dt) {   /* moleculemover.cpp:37-41 */
@@ -707,6 +730,7 @@ static void move_molecule(or_system *s, long mi, double dt_in) {
                     { double m = face_margin(s, r); if (m < *mg) *mg = m; }
                 } else {
                     dt -= tau;
+                    { double m = time_margin(s, v, dt); if (m < *mg) *mg = m; }
                     resumed = 1;
                     break;
                 }
@@ -727,8 +751,11 @@ static void move_molecule(or_system *s, long mi, double dt_in) {
                 do_move(r, v, -tau);
                 int no_face = 0;
                 double em = 1e300;
-                tau = time_until_collision(s, r, v, cvi, &no_face, &em);
+                int face = -1;
+                double raw = 0;
+                tau = time_until_collision(s, r, v, cvi, &no_face, &em, &face, &raw);
                 if (em < *mg) *mg = em;
+                if (!no_face) { double m = time_margin(s, v, raw); if (m < *mg) *mg = m; }
                 if (no_face) {
                     /* The reference would jump 1e9 time units and index out
                      * of its grid; the oracle (and the GPU) count an error
@@ -740,9 +767,14 @@ static void move_molecule(or_system *s, long mi, double dt_in) {
                 }
                 do_move(r, v, tau);
                 idx = or_voxel_index_halo(s, r);
-                { double m = face_margin(s, r); if (m < *mg) *mg = m; }
+                /* the landing stands 1e-7 time units before the entry face on
+                 * purpose; only the other two axes are decisions.  A zero
+                 * face time leaves the particle at its flight's start point:
+                 * no new decision. */
+                if (face >= 0) { double m = face_margin_excl(s, r, face / 2); if (m < *mg) *mg = m; }
             }
             dt -= tau;
+            { double m = time_margin(s, v, dt); if (m < *mg) *mg = m; }
             wall_collide(s, v, cvi, mi, s->cur_hit++);
         } else {
             dt = 0;

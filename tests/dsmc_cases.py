@@ -44,6 +44,16 @@ def cell_of_positions(rv6, ini):
     return (ijk[:, 0] * c + ijk[:, 1]) * c + ijk[:, 2]
 
 
+def vel_err(a, b, ini):
+    """|a - b| / max(|b|, sqrt(T/m)): relative error with the thermal speed
+    as the floor (a slow particle's velocity carries the rounding of the
+    larger speeds it was computed from)."""
+    T, m, _ = C.gas_scales(ini)
+    floor = np.sqrt(T / m)
+    d = np.linalg.norm(a - b, axis=1)
+    return d / np.maximum(np.linalg.norm(b, axis=1), floor)
+
+
 def rel_vec_err(a, b):
     """|a - b| / max(|b|, tiny) per row."""
     d = np.linalg.norm(a - b, axis=1)

@@ -63,8 +63,12 @@ def test_move_and_sort_match_oracle(gpu, wall, mult):
     nh_g, lv_g = g.last_wall_hits()
     nh_o, lv_o = o.last_wall()
     ok = ~amb
-    assert np.array_equal(nh_g[ok], nh_o[ok])
-    assert np.array_equal(lv_g[ok], lv_o[ok])
+    bad = np.nonzero(ok & ((nh_g != nh_o) | (lv_g != lv_o)))[0]
+    if len(bad):
+        for i in bad[:10]:
+            print(f"mismatch p={i} hits gpu={nh_g[i]} oracle={nh_o[i]} voxel gpu={lv_g[i]} "
+                  f"oracle={lv_o[i]} margin={margin[i]:.3e} r0={st0[i, :3]} v0={st0[i, 3:]}")
+    assert len(bad) == 0, f"{len(bad)} unambiguous particles differ in wall hits"
     assert nh_o.sum() > 100
     # permutation: identical when no decision is ambiguous, else identical
     # restricted to unambiguous particles (same cell, same relative order)
@@ -81,8 +85,20 @@ def test_move_and_sort_match_oracle(gpu, wall, mult):
         h = float(ini["Lx"]) / 32
         dr = np.abs(a[:, :3] - b[:, :3]).max(axis=1)
         assert np.all(dr <= 1e-6 * np.maximum(np.abs(b[:, :3]).max(axis=1), h))
-        assert K.rel_vec_err(a[:, 3:], b[:, 3:]).max() <= 1e-6
+        assert K.vel_err(a[:, 3:], b[:, 3:], ini).max() <= 1e-6
         assert np.array_equal(K.cell_of_positions(a, ini), K.cell_of_positions(b, ini))
+    else:
+        # same comparison on the particles whose decisions are unambiguous
+        inv = np.empty_like(po)
+        inv[po] = np.arange(len(po))
+        invg = np.empty_like(pg)
+        invg[pg] = np.arange(len(pg))
+        keep = np.nonzero(~amb)[0]
+        ga, ob = a[invg[keep]], b[inv[keep]]
+        h = float(ini["Lx"]) / 32
+        dr = np.abs(ga[:, :3] - ob[:, :3]).max(axis=1)
+        assert np.all(dr <= 1e-6 * np.maximum(np.abs(ob[:, :3]).max(axis=1), h))
+        assert K.vel_err(ga[:, 3:], ob[:, 3:], ini).max() <= 1e-6
     cg, co = g.counters(), o.counters()
     if not amb.any():
         assert cg["wall_collisions"] == co["wall_collisions"]
@@ -113,7 +129,9 @@ def test_collide_matches_oracle(gpu):
     cell = K.cell_of_positions(b, ini)
     bad_cells = np.nonzero(act)[0][bad]
     keep = ~np.isin(cell, bad_cells)
-    assert K.rel_vec_err(a[keep, 3:], b[keep, 3:]).max() <= 1e-6
+    # FP32 collisions: each accepted pair adds a few ulp of the pair's speed
+    # scale; a particle collides a handful of times per step at most
+    assert K.vel_err(a[keep, 3:], b[keep, 3:], ini).max() <= 2e-6
     good = np.setdiff1d(np.nonzero(act)[0], bad_cells)
     assert np.allclose(gc["vr_max"][good], oc["vr_max"][good], rtol=1e-6)
     assert ag.sum() > 10000
@@ -150,7 +168,7 @@ def test_collision_conservation(gpu):
 
 
 def test_specular_preserves_speed(gpu):
-    ini, w, g, o, st0 = _init_pair(gpu, wall="specular", dt_tau=2.0)
+    ini, w, g, o, st0 = _init_pair(gpu, wall="specular", dt_tau=2.0, gravity=0.0)
     g.set_option(K.dsmc.OPT_PHASES, 1)
     g.set_option(K.dsmc.OPT_RECORD, 1)
     g.step(1)
@@ -160,7 +178,10 @@ def test_specular_preserves_speed(gpu):
     s1 = np.linalg.norm(a[:, 3:], axis=1)
     nh, _ = g.last_wall_hits()
     assert nh.sum() > 1000
-    assert np.allclose(s1, s0, rtol=2e-6)
+    # the world's frames are orthonormal to ~4e-7 (float Gram-Schmidt,
+    # complexgeometry.cpp:1014-1042): each reflection may change |v| by ~1e-6
+    tol = 1e-6 * (1 + 2 * nh[perm])
+    assert np.all(np.abs(s1 - s0) <= tol * s0)
 
 
 def test_graph_replay_is_bit_identical(gpu):
