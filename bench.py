@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--lds-cap", type=int, default=0, help="particles per cell held in LDS (0: automatic)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the DSMC step here is a single-GPU path "
@@ -108,7 +109,10 @@ def main():
     g = dsmc.System(ini, w)
     n = g.initialize()
     g.set_sampling(1)
-    log(f"[bench] {n} particles seeded+sorted ({time.time() - t_setup:.1f}s)")
+    if args.lds_cap:
+        g.set_option(dsmc.OPT_LDS_CAPACITY, args.lds_cap)
+    log(f"[bench] {n} particles seeded+sorted ({time.time() - t_setup:.1f}s), "
+        f"max cell occupancy {g.counters()['max_cell_occupancy']}")
 
     # per-kernel device times (HIP events on the handle's stream), eager launches
     g.set_option(dsmc.OPT_TIMING, 1)
