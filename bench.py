@@ -96,7 +96,19 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--lds-cap", type=int, default=0, help="particles per cell held in LDS (0: automatic)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device-to-device copy (rocprof counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28                                 # 2^28 float32 = 1 GiB
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the DSMC step here is a single-GPU path "
               "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)

@@ -340,7 +340,9 @@ struct Scope {
 };
 
 uint32_t auto_lds_cap(int64_t max_occ) {
-    int64_t k = (int64_t)(max_occ * 1.25) + 64;
+    // LDS per cell bounds the collide kernel's occupancy: a small margin over
+    // the largest cell; cells that outgrow it take the global-memory path
+    int64_t k = max_occ + 64;
     k = (k + 63) / 64 * 64;
     if (k < 128) k = 128;
     if (k > 4096) k = 4096;
@@ -393,6 +395,33 @@ GatherArgs gather_args(dsmc_handle *h, bool collide, bool sample) {
     return g;
 }
 
+// Collide on the buffer that holds the freshly sorted state (h->cur after the
+// gather's swap).
+CollideArgs collide_args(dsmc_handle *h, bool collide, bool sample) {
+    CollideArgs c;
+    std::memset(&c, 0, sizeof c);
+    int b = h->cur;
+    c.vx = h->vx[b]; c.vy = h->vy[b]; c.vz = h->vz[b];
+    c.start = h->d_start[h->scur];
+    c.members = h->d_members;
+    c.cell_member = h->d_cell_member;
+    c.coeff = h->d_coeff;
+    c.vr_max = h->d_vrmax;
+    c.rest = h->d_rest;
+    c.stat_pairs = h->d_pairs;
+    c.stat_acc = h->d_acc;
+    c.moments = h->d_mom;
+    c.scratch = h->d_mv_cell;
+    c.step = h->d_step;
+    c.counters = h->d_cnt;
+    c.ncells = h->ncells;
+    c.cell_begin = 0;
+    c.lds_cap = h->lds_cap;
+    c.do_collide = collide ? 1 : 0;
+    c.do_sample = sample ? 1 : 0;
+    return c;
+}
+
 ScanArgs scan_args(dsmc_handle *h) {
     ScanArgs s;
     s.start = h->d_start[h->scur];
@@ -442,12 +471,7 @@ dsmc_status sort_unsorted(dsmc_handle *h, const std::vector<uint32_t> &part_memb
     HIPCHK(h, launch_classify_all(c, s));
     HIPCHK(h, launch_scan(scan_args(h), s));
     HIPCHK(h, launch_bucket(bucket_args(h), s));
-    // LDS holds the largest cell; at this point the occupancy is not known
-    // yet, so the overflow path handles anything above the default.
-    uint32_t keep = h->lds_cap;
-    if (h->lds_cap == 0) h->lds_cap = 1024;
-    HIPCHK(h, launch_gather_collide(gather_args(h, false, false), h->ncells, s));
-    h->lds_cap = keep;
+    HIPCHK(h, launch_gather(gather_args(h, false, false), h->ncells, s));
     h->cur = 1 - h->cur;
     h->scur = 1 - h->scur;
     HIPCHK(h, hipStreamSynchronize(s));
@@ -497,16 +521,21 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
         HIPCHK(h, launch_bucket_cells(bucket_args(h), s));
     }
     {
-        Scope sc(h, "k_gather_collide");
-        HIPCHK(h, launch_gather_collide(gather_args(h, (h->phases & 4) != 0, sample), h->ncells, s));
-    }
-    if (h->totals) {
-        int b = 1 - h->cur;
-        HIPCHK(h, launch_totals_partial(h->vx[b], h->vy[b], h->vz[b], (uint32_t)h->N,
-                                        h->d_partial + TOT_BLOCKS * 4, TOT_BLOCKS, s));
+        Scope sc(h, "k_gather");
+        HIPCHK(h, launch_gather(gather_args(h, false, false), h->ncells, s));
     }
     h->cur = 1 - h->cur;
     h->scur = 1 - h->scur;
+    const bool collide = (h->phases & 4) != 0;
+    if (collide || sample) {
+        Scope sc(h, "k_collide");
+        HIPCHK(h, launch_collide(collide_args(h, collide, sample), h->ncells, s));
+    }
+    if (h->totals) {
+        int b = h->cur;
+        HIPCHK(h, launch_totals_partial(h->vx[b], h->vy[b], h->vz[b], (uint32_t)h->N,
+                                        h->d_partial + TOT_BLOCKS * 4, TOT_BLOCKS, s));
+    }
     return DSMC_OK;
 }
 

@@ -73,6 +73,22 @@ struct GatherArgs {
     int do_collide, do_sample;
 };
 
+struct CollideArgs {
+    float *vx, *vy, *vz;                // state arrays, cell-sorted
+    const uint32_t *start;
+    const Member *members;
+    const uint32_t *cell_member;
+    const double *coeff;
+    double *vr_max, *rest;
+    int32_t *stat_pairs, *stat_acc;
+    double *moments;                    // 7 x ncells (SoA)
+    uint32_t *scratch;                  // per-particle marks for cells beyond lds_cap
+    const uint32_t *step;
+    unsigned long long *counters;
+    uint32_t ncells, cell_begin, lds_cap;
+    int do_collide, do_sample;
+};
+
 struct SeedArgs {
     float *ox, *oy, *oz, *vx, *vy, *vz;
     uint32_t *vox;
@@ -89,7 +105,8 @@ hipError_t launch_classify_all(const ClassifyArgs &a, hipStream_t s);
 hipError_t launch_scan(const ScanArgs &a, hipStream_t s);
 hipError_t launch_bucket(const BucketArgs &a, hipStream_t s);
 hipError_t launch_bucket_cells(const BucketArgs &a, hipStream_t s);
-hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, hipStream_t s);
+hipError_t launch_gather(const GatherArgs &a, uint32_t ncells, hipStream_t s);
+hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_begin_step(uint32_t *step, uint32_t *mv_count, hipStream_t s);

@@ -5,11 +5,12 @@
 //                     current cell-sorted order; classifies every particle as
 //                     staying in its cell or moving to another (mover list)
 //   k_scan_*          new cell counts and their exclusive scans
-//   k_bucket          movers into per-destination-cell buckets
-//   k_gather_collide  per destination cell: stable gather (arrivals from lower
+//   k_bucket_cells    movers into per-destination-cell buckets
+//   k_gather          per destination cell: stable gather (arrivals from lower
 //                     old indices, stayers, arrivals from higher ones) into
-//                     the other state buffer, NTC collisions in LDS,
-//                     per-cell FP64 moments, write-out
+//                     the other state buffer
+//   k_collide         per cell: NTC collisions on the cell's velocities in
+//                     LDS, per-cell FP64 moments, write-back
 // The stable order is the one of a stable counting sort keyed by cell
 // (matlab/3d/sortParticles.m:11-63): inside a cell, particles keep the order
 // of their indices before the step.
@@ -264,174 +265,87 @@ __global__ __launch_bounds__(256) void k_bucket_cells(BucketArgs a) {
     }
 }
 
-// ------------------------------------------------------------------ gather + collide
-// Velocity storage for one cell: LDS when it fits, else the cell's range of
-// the output arrays (global memory, owned by this wave).
-struct VelStore {
-    float *x, *y, *z;
-    uint32_t *mark;   // LDS conflict marks (nullptr on the global path)
-};
+// ------------------------------------------------------------------ gather
+// Per destination cell: arrivals from lower old indices, then the cell's
+// stayers in their old order, then arrivals from higher old indices -- the
+// order of a stable counting sort by cell.  No LDS beyond a small key buffer,
+// so the kernel runs at full occupancy.
+static constexpr int GATHER_KEYS = 512;
+static constexpr int GATHER_BATCH = 8;      // chunks of 64 particles per round trip
 
-__device__ __forceinline__ void ntc_collide(const Member &M, uint32_t cell_local, uint32_t step,
-                                            uint32_t n, int pairs, double vrm, VelStore S,
-                                            double &crm_out, uint32_t &accepted_out,
-                                            uint32_t &cand_out) {
-    const uint32_t lane = threadIdx.x;
-    double crm = vrm;
-    uint32_t acc = 0, cand = 0;
-    if (S.mark)
-        for (uint32_t i = lane; i < n; i += WAVE) S.mark[i] = 0xffffffffu;
-    __syncthreads();
-    while ((int)cand < pairs) {
-        const uint32_t k = cand + lane;
-        const bool valid = (int)k < pairs;
-        const uint4 o = philox(TAG_SELECT, step, cell_local, k, M.key_lo, M.key_hi);
-        const uint32_t i0 = __umulhi(o.x, n);
-        const uint32_t i1 = (i0 + 1u + __umulhi(o.y, n - 1u)) % n;
-        const float dx = S.x[i0] - S.x[i1];
-        const float dy = S.y[i0] - S.y[i1];
-        const float dz = S.z[i0] - S.z[i1];
-        const float vrel = sqrtf(dx * dx + dy * dy + dz * dz);
-        const bool accept = valid && (double)vrel > u01d(o.z) * vrm;
-        bool conflict;
-        if (S.mark) {
-            if (accept) {
-                atomicMin(&S.mark[i0], lane);
-                atomicMin(&S.mark[i1], lane);
-            }
-            __syncthreads();
-            conflict = valid && (S.mark[i0] < lane || S.mark[i1] < lane);
-        } else {
-            conflict = false;
-            for (uint32_t j = 0; j < WAVE; j++) {
-                uint32_t a0 = __shfl(i0, (int)j, WAVE), a1 = __shfl(i1, (int)j, WAVE);
-                int acj = __shfl((int)accept, (int)j, WAVE);
-                if (j < lane && acj && (a0 == i0 || a0 == i1 || a1 == i0 || a1 == i1)) conflict = true;
-            }
-            conflict = conflict && valid;
-        }
-        const uint64_t cmask = __ballot(conflict);
-        const uint32_t remaining = (uint32_t)(pairs - (int)cand);
-        const uint32_t f = cmask ? (uint32_t)(__ffsll((long long)cmask) - 1) :
-                                   (remaining < (uint32_t)WAVE ? remaining : (uint32_t)WAVE);
-        const bool commit = lane < f;
-        if (commit && (double)vrel > crm) crm = (double)vrel;
-        const bool do_scatter = commit && accept;
-        acc += (uint32_t)__popcll(__ballot(do_scatter));
-        if (do_scatter) {
-            const uint4 q = philox(TAG_SCATTER, step, cell_local, k, M.key_lo, M.key_hi);
-            const float cos_th = 1.0f - 2.0f * u01f(q.x);
-            const float sin_th = sqrtf(1.0f - cos_th * cos_th);
-            float sp, cp;
-            sincospif(2.0f * u01f(q.y), &sp, &cp);
-            const float vrx = vrel * cos_th;
-            const float vry = vrel * sin_th * cp;
-            const float vrz = vrel * sin_th * sp;
-            const float cmx = 0.5f * (S.x[i0] + S.x[i1]);
-            const float cmy = 0.5f * (S.y[i0] + S.y[i1]);
-            const float cmz = 0.5f * (S.z[i0] + S.z[i1]);
-            S.x[i0] = cmx + 0.5f * vrx; S.y[i0] = cmy + 0.5f * vry; S.z[i0] = cmz + 0.5f * vrz;
-            S.x[i1] = cmx - 0.5f * vrx; S.y[i1] = cmy - 0.5f * vry; S.z[i1] = cmz - 0.5f * vrz;
-        }
-        __syncthreads();
-        if (S.mark && accept) {
-            S.mark[i0] = 0xffffffffu;
-            S.mark[i1] = 0xffffffffu;
-        }
-        __syncthreads();
-        cand += f;
-    }
-    crm_out = wave_max(crm);
-    accepted_out = acc;
-    cand_out = cand;
-}
-
-extern __shared__ float dyn_lds[];
-
-__global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
+__global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
     const uint32_t c = blockIdx.x + a.cell_begin;
     const uint32_t lane = threadIdx.x;
     const uint32_t ns = a.start_new[c], ne = a.start_new[c + 1];
     const uint32_t n = ne - ns;
+    if (n == 0) return;
     const uint32_t os = a.start[c], oe = a.start[c + 1];
     const uint32_t bs = a.bstart[c], be = a.bstart[c + 1];
     const uint32_t nb = be - bs;
     const uint32_t n_stay = n - nb;
-    const uint32_t K = a.lds_cap;
-    const bool in_lds = n <= K;
-    float *lx = dyn_lds, *ly = dyn_lds + K, *lz = dyn_lds + 2 * K;
-    uint32_t *lmark = (uint32_t *)(dyn_lds + 3 * K);   // also the arrival-key buffer
-    VelStore S;
-    if (in_lds) { S.x = lx; S.y = ly; S.z = lz; S.mark = lmark; }
-    else { S.x = a.vx2 + ns; S.y = a.vy2 + ns; S.z = a.vz2 + ns; S.mark = nullptr; }
-
-    // ---- arrivals: rank by index before the step (stable order)
-    uint32_t n_low = 0;
-    const bool keys_lds = nb <= K;
-    if (nb > WAVE && keys_lds) {
-        for (uint32_t i = lane; i < nb; i += WAVE) lmark[i] = a.bucket[bs + i];
-        __syncthreads();
-    }
-    for (uint32_t i = lane; i < nb; i += WAVE) n_low += a.bucket[bs + i] < os ? 1u : 0u;
-    n_low = wave_sum(n_low);
-
-    const uint32_t step = *a.step;
+    __shared__ uint32_t keys[GATHER_KEYS];
     const Member M = a.members[a.cell_member[c]];
 
-    // ---- stayers: stable compaction of the old range by wave ballot; all
-    // seven arrays of a chunk are loaded together and the next chunk is in
-    // flight while this one is placed
+    // arrivals: keys (indices before the step) and how many precede the stayers
+    uint32_t n_low = 0;
+    const bool keys_lds = nb > WAVE && nb <= GATHER_KEYS;
+    for (uint32_t i = lane; i < nb; i += WAVE) {
+        const uint32_t k = a.bucket[bs + i];
+        if (keys_lds) keys[i] = k;
+        n_low += k < os ? 1u : 0u;
+    }
+    n_low = wave_sum(n_low);
+    if (keys_lds) __syncthreads();
+
+    // stayers: stable compaction by wave ballot; the seven arrays of
+    // GATHER_BATCH chunks are loaded per round trip
     uint32_t run = 0;
-    uint32_t p = os + lane;
-    float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
-    uint32_t vp = 0;
-    if (p < oe) {
-        vp = a.vox[p];
-        ox = a.ox[p]; oy = a.oy[p]; oz = a.oz[p];
-        vx = a.vx[p]; vy = a.vy[p]; vz = a.vz[p];
+    for (uint32_t base = os; base < oe; base += GATHER_BATCH * WAVE) {
+        float ox[GATHER_BATCH], oy[GATHER_BATCH], oz[GATHER_BATCH], vx[GATHER_BATCH], vy[GATHER_BATCH], vz[GATHER_BATCH];
+        uint32_t vp[GATHER_BATCH];
+#pragma unroll
+        for (int q = 0; q < GATHER_BATCH; q++) {
+            const uint32_t p = base + q * WAVE + lane;
+            vp[q] = 0xffffffffu;
+            if (p < oe) {
+                vp[q] = a.vox[p];
+                ox[q] = a.ox[p]; oy[q] = a.oy[p]; oz[q] = a.oz[p];
+                vx[q] = a.vx[p]; vy[q] = a.vy[p]; vz[q] = a.vz[p];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < GATHER_BATCH; q++) {
+            const uint32_t p = base + q * WAVE + lane;
+            bool stay = false;
+            if (p < oe) {
+                int ix, iy, iz;
+                unpack_vox(vp[q], ix, iy, iz);
+                stay = cell_of(M, ix, iy, iz) == c;
+            }
+            const uint64_t sm = __ballot(stay);
+            if (stay) {
+                const uint32_t d = ns + n_low + run + mbcnt(sm);
+                a.ox2[d] = ox[q]; a.oy2[d] = oy[q]; a.oz2[d] = oz[q];
+                a.vx2[d] = vx[q]; a.vy2[d] = vy[q]; a.vz2[d] = vz[q];
+                a.vox2[d] = vp[q];
+                if (a.rec_perm) a.rec_perm[d] = (int32_t)p;
+            }
+            run += (uint32_t)__popcll(sm);
+        }
     }
-    for (uint32_t base = os; base < oe; base += WAVE) {
-        const uint32_t pn = p + WAVE;
-        float nox = 0, noy = 0, noz = 0, nvx = 0, nvy = 0, nvz = 0;
-        uint32_t nvp = 0;
-        if (pn < oe) {
-            nvp = a.vox[pn];
-            nox = a.ox[pn]; noy = a.oy[pn]; noz = a.oz[pn];
-            nvx = a.vx[pn]; nvy = a.vy[pn]; nvz = a.vz[pn];
-        }
-        const bool valid = p < oe;
-        bool stay = false;
-        if (valid) {
-            int ix, iy, iz;
-            unpack_vox(vp, ix, iy, iz);
-            stay = cell_of(M, ix, iy, iz) == c;
-        }
-        const uint64_t sm = __ballot(stay);
-        if (stay) {
-            const uint32_t L = n_low + run + mbcnt(sm);
-            const uint32_t d = ns + L;
-            a.ox2[d] = ox; a.oy2[d] = oy; a.oz2[d] = oz;
-            a.vox2[d] = vp;
-            S.x[L] = vx; S.y[L] = vy; S.z[L] = vz;
-            if (a.rec_perm) a.rec_perm[d] = (int32_t)p;
-        }
-        run += (uint32_t)__popcll(sm);
-        p = pn;
-        ox = nox; oy = noy; oz = noz; vx = nvx; vy = nvy; vz = nvz; vp = nvp;
-    }
-    // ---- arrivals
+    // arrivals: rank by index before the step (keys are distinct)
     for (uint32_t base = 0; base < nb; base += WAVE) {
         const uint32_t i = base + lane;
         const bool valid = i < nb;
-        uint32_t key = valid ? a.bucket[bs + i] : 0xffffffffu;
+        const uint32_t key = valid ? a.bucket[bs + i] : 0xffffffffu;
         uint32_t rank = 0;
         if (nb <= WAVE) {
             for (uint32_t j = 0; j < nb; j++) {
-                uint32_t kj = (uint32_t)__shfl((int)key, (int)j, WAVE);
+                const uint32_t kj = (uint32_t)__shfl((int)key, (int)j, WAVE);
                 rank += kj < key ? 1u : 0u;
             }
         } else if (keys_lds) {
-            for (uint32_t j = 0; j < nb; j++) rank += lmark[j] < key ? 1u : 0u;
+            for (uint32_t j = 0; j < nb; j++) rank += keys[j] < key ? 1u : 0u;
         } else {
             for (uint32_t j = 0; j < nb; j++) rank += a.bucket[bs + j] < key ? 1u : 0u;
         }
@@ -440,52 +354,314 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
             const uint32_t d = ns + L;
             const uint32_t q = key;
             a.ox2[d] = a.ox[q]; a.oy2[d] = a.oy[q]; a.oz2[d] = a.oz[q];
+            a.vx2[d] = a.vx[q]; a.vy2[d] = a.vy[q]; a.vz2[d] = a.vz[q];
             a.vox2[d] = a.vox[q];
-            S.x[L] = a.vx[q]; S.y[L] = a.vy[q]; S.z[L] = a.vz[q];
             if (a.rec_perm) a.rec_perm[d] = (int32_t)q;
         }
     }
-    __syncthreads();
+}
 
-    // ---- collide (Cell::prepare + Cell::collide, cell.cpp:42-111)
-    const bool active = a.coeff[c] > 0.0;
-    if (a.do_collide && active) {
-        const double vrm = a.vr_max[c];
-        const double select = a.coeff[c] * (double)n * (double)((int)n - 1) * vrm + a.rest[c];
-        const int pairs = (int)round(select);
-        const double rest = select - (double)pairs;
-        double crm = vrm;
-        uint32_t acc = 0, cand = 0;
-        if (n >= 2) ntc_collide(M, c - M.cell0, step, n, pairs, vrm, S, crm, acc, cand);
-        if (lane == 0) {
-            a.rest[c] = rest;
-            if (n >= 2) a.vr_max[c] = crm;
-            a.stat_pairs[c] = pairs;
-            a.stat_acc[c] = (int32_t)acc;
-            if (acc) atomicAdd(&a.counters[CNT_COLL], (unsigned long long)acc);
-            if (cand) atomicAdd(&a.counters[CNT_CAND], (unsigned long long)cand);
-            if (!in_lds) atomicAdd(&a.counters[CNT_OVERFLOW], 1ull);
+// ------------------------------------------------------------------ collide
+// Velocity and mark storage of one cell: LDS when the cell fits, else the
+// cell's own range of the state arrays plus a per-particle scratch array in
+// global memory (accessed coherently, the wave owns the range).
+struct LdsStore {
+    float *x, *y, *z;
+    uint32_t *mark;
+    __device__ __forceinline__ float lx(uint32_t i) const { return x[i]; }
+    __device__ __forceinline__ float ly(uint32_t i) const { return y[i]; }
+    __device__ __forceinline__ float lz(uint32_t i) const { return z[i]; }
+    __device__ __forceinline__ void st(uint32_t i, float a, float b, float c) const { x[i] = a; y[i] = b; z[i] = c; }
+    __device__ __forceinline__ void mark_min(uint32_t i, uint32_t v) const { atomicMin(&mark[i], v); }
+    __device__ __forceinline__ uint32_t mark_ld(uint32_t i) const { return mark[i]; }
+    __device__ __forceinline__ void mark_st(uint32_t i, uint32_t v) const { mark[i] = v; }
+};
+
+struct GlobalStore {
+    float *x, *y, *z;
+    uint32_t *mark;
+    __device__ __forceinline__ static float ldf(float *p) {
+        return __uint_as_float(__hip_atomic_load((uint32_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    __device__ __forceinline__ static void stf(float *p, float v) {
+        __hip_atomic_store((uint32_t *)p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ float lx(uint32_t i) const { return ldf(x + i); }
+    __device__ __forceinline__ float ly(uint32_t i) const { return ldf(y + i); }
+    __device__ __forceinline__ float lz(uint32_t i) const { return ldf(z + i); }
+    __device__ __forceinline__ void st(uint32_t i, float a, float b, float c) const {
+        stf(x + i, a); stf(y + i, b); stf(z + i, c);
+    }
+    __device__ __forceinline__ void mark_min(uint32_t i, uint32_t v) const { atomicMin(&mark[i], v); }
+    __device__ __forceinline__ uint32_t mark_ld(uint32_t i) const {
+        return __hip_atomic_load(mark + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void mark_st(uint32_t i, uint32_t v) const {
+        __hip_atomic_store(mark + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// Cell::collide (cell.cpp:75-111) for one cell, candidates in their
+// sequential order.  A wave evaluates up to 64 candidates at once; a
+// candidate commits unless an earlier candidate of the batch that touches
+// one of its particles either collides now or is itself still pending (its
+// outcome unknown).  Pending candidates keep their draws and move to the
+// front of the next batch; new candidates fill the remaining lanes.  The
+// result is the sequential NTC loop's, for every interleaving.
+template <class Store>
+__device__ __forceinline__ void ntc_cell(const Member &M, uint32_t cell_local, uint32_t step, uint32_t n,
+                                         int pairs, double vrm, const Store &S, uint32_t *stage,
+                                         double &crm_out, uint32_t &acc_out) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t NONE = 0xffffffffu;
+    for (uint32_t i = lane; i < n; i += WAVE) S.mark_st(i, NONE);
+    __syncthreads();
+    double crm = vrm;
+    uint32_t acc = 0;
+    const uint32_t npairs = pairs > 0 ? (uint32_t)pairs : 0u;
+    uint32_t next = 0;       // next candidate index not yet in a lane
+    uint32_t P = 0;          // pending candidates held in lanes [0, P)
+    uint32_t k = NONE, i0 = 0, i1 = 0, zc = 0, zp = 0;
+    float u = 0.0f;
+    for (;;) {
+        // fill lanes [P, 64) with new candidates, in order.  One Philox call
+        // per candidate: word 0 picks the first particle (its product's low
+        // half, uniform and independent of the pick, the offset of the
+        // second), word 1 the acceptance, words 2-3 the scattering angles.
+        if (lane >= P) {
+            const uint32_t kk = next + (lane - P);
+            if (kk < npairs) {
+                const uint4 o = philox(TAG_SELECT, step, cell_local, kk, M.key_lo, M.key_hi);
+                k = kk;
+                i0 = __umulhi(o.x, n);
+                i1 = i0 + 1u + __umulhi(o.x * n, n - 1u);
+                if (i1 >= n) i1 -= n;
+                u = u01f(o.y);
+                zc = o.z;
+                zp = o.w;
+            } else {
+                k = NONE;
+            }
+        }
+        {
+            const uint32_t room = (uint32_t)WAVE - P;
+            next = (npairs - next < room) ? npairs : next + room;
+        }
+        const bool valid = k != NONE;
+        if (__ballot(valid) == 0ull) break;
+        float vrel = 0.0f;
+        bool accept = false;
+        if (valid) {
+            const float dx = S.lx(i0) - S.lx(i1);
+            const float dy = S.ly(i0) - S.ly(i1);
+            const float dz = S.lz(i0) - S.lz(i1);
+            vrel = sqrtf(dx * dx + dy * dy + dz * dz);
+            accept = (double)vrel > (double)u * vrm;
+        }
+        // blockers: accepted candidates, then (closure) pending ones
+        bool blocker = accept, pending = false, marked = accept;
+        if (accept) { S.mark_min(i0, lane); S.mark_min(i1, lane); }
+        for (;;) {
+            __syncthreads();
+            const bool dep = valid && (S.mark_ld(i0) < lane || S.mark_ld(i1) < lane);
+            pending = pending || dep;
+            const bool add = pending && !marked;
+            if (__ballot(add) == 0ull) break;
+            if (add) { S.mark_min(i0, lane); S.mark_min(i1, lane); marked = true; }
+        }
+        (void)blocker;
+        const bool commit = valid && !pending;
+        if (commit && (double)vrel > crm) crm = (double)vrel;
+        const bool scatter = commit && accept;
+        acc += (uint32_t)__popcll(__ballot(scatter));
+        __syncthreads();
+        if (scatter) {
+            const float cos_th = 1.0f - 2.0f * u01f(zc);
+            const float sin_th = sqrtf(1.0f - cos_th * cos_th);
+            // phi = 2 pi u: the hardware sine/cosine take their argument in
+            // revolutions, so u enters exactly (max abs error 1.3e-7)
+            const float rev = u01f(zp);
+            const float sp = __builtin_amdgcn_sinf(rev), cp = __builtin_amdgcn_cosf(rev);
+            const float vrx = vrel * cos_th;
+            const float vry = vrel * sin_th * cp;
+            const float vrz = vrel * sin_th * sp;
+            const float ax = S.lx(i0), ay = S.ly(i0), az = S.lz(i0);
+            const float bx = S.lx(i1), by = S.ly(i1), bz = S.lz(i1);
+            const float cmx = 0.5f * (ax + bx), cmy = 0.5f * (ay + by), cmz = 0.5f * (az + bz);
+            S.st(i0, cmx + 0.5f * vrx, cmy + 0.5f * vry, cmz + 0.5f * vrz);
+            S.st(i1, cmx - 0.5f * vrx, cmy - 0.5f * vry, cmz - 0.5f * vrz);
+        }
+        if (marked) { S.mark_st(i0, NONE); S.mark_st(i1, NONE); }
+        // pending candidates to the front, in order
+        const uint64_t pm = __ballot(pending);
+        const uint32_t np = (uint32_t)__popcll(pm);
+        if (pending) {
+            const uint32_t r = mbcnt(pm);
+            stage[r] = k;
+            stage[64 + r] = i0;
+            stage[128 + r] = i1;
+            stage[192 + r] = __float_as_uint(u);
+            stage[256 + r] = zc;
+            stage[320 + r] = zp;
         }
         __syncthreads();
-    }
-
-    // ---- moments (FP64) and write-out
-    double m[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t i = lane; i < n; i += WAVE) {
-        const float x = S.x[i], y = S.y[i], z = S.z[i];
-        if (!in_lds) { /* already in place */ }
-        else { a.vx2[ns + i] = x; a.vy2[ns + i] = y; a.vz2[ns + i] = z; }
-        const double dx = x, dy = y, dz = z;
-        m[0] += 1.0; m[1] += dx; m[2] += dy; m[3] += dz;
-        m[4] += dx * dx; m[5] += dy * dy; m[6] += dz * dz;
-    }
-    if (a.do_sample) {
-#pragma unroll
-        for (int j = 0; j < 7; j++) m[j] = wave_sum(m[j]);
-        if (lane == 0) {
-#pragma unroll
-            for (int j = 0; j < 7; j++) a.moments[(size_t)j * a.ncells + c] += m[j];
+        if (lane < np) {
+            k = stage[lane];
+            i0 = stage[64 + lane];
+            i1 = stage[128 + lane];
+            u = __uint_as_float(stage[192 + lane]);
+            zc = stage[256 + lane];
+            zp = stage[320 + lane];
         }
+        P = np;
+        __syncthreads();
+    }
+    crm_out = wave_max(crm);
+    acc_out = acc;
+}
+
+extern __shared__ float dyn_lds[];
+
+// Per-cell values the collide loop needs before it can start.
+struct CellMeta {
+    uint32_t c, s, n;
+    double coeff, vrm, rest;
+    uint32_t member;
+    bool valid;
+};
+
+__device__ __forceinline__ CellMeta load_meta(const CollideArgs &a, uint32_t c) {
+    CellMeta m;
+    m.c = c;
+    m.valid = c < a.ncells;
+    m.s = 0; m.n = 0; m.coeff = 0.0; m.vrm = 0.0; m.rest = 0.0; m.member = 0;
+    if (m.valid) {
+        m.s = a.start[c];
+        m.n = a.start[c + 1] - m.s;
+        m.coeff = a.coeff[c];
+        m.vrm = a.vr_max[c];
+        m.rest = a.rest[c];
+        m.member = a.cell_member[c];
+    }
+    return m;
+}
+
+static constexpr int PREFETCH_CHUNKS = 16;   // velocities of up to 1024 particles in registers
+
+// Persistent waves: wave w handles cells w, w + G, w + 2G, ... (G waves in
+// the grid).  While the NTC loop of one cell runs on LDS, the velocities of
+// the next cell are already in flight into registers and the metadata of
+// the one after is loading, so the loop's global-memory latency is hidden.
+__global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t G = gridDim.x;
+    const uint32_t K = a.lds_cap;
+    float *lx = dyn_lds, *ly = dyn_lds + K, *lz = dyn_lds + 2 * K;
+    uint32_t *lmark = (uint32_t *)(dyn_lds + 3 * K);
+    uint32_t *stage = lmark + K;
+    const uint32_t step = *a.step;
+    unsigned long long acc_total = 0, cand_total = 0, overflow = 0;
+
+    CellMeta m0 = load_meta(a, blockIdx.x);
+    CellMeta m1 = load_meta(a, blockIdx.x + G);
+    float px[PREFETCH_CHUNKS], py[PREFETCH_CHUNKS], pz[PREFETCH_CHUNKS];
+    double pmom = 0.0;
+#define DSMC_PREFETCH(M)                                                                    \
+    do {                                                                                    \
+        _Pragma("unroll") for (int q = 0; q < PREFETCH_CHUNKS; q++) {                       \
+            const uint32_t i = q * WAVE + lane;                                             \
+            if ((M).valid && (M).coeff > 0.0 && i < (M).n) {                                \
+                px[q] = a.vx[(M).s + i]; py[q] = a.vy[(M).s + i]; pz[q] = a.vz[(M).s + i];  \
+            }                                                                               \
+        }                                                                                   \
+        pmom = 0.0;                                                                         \
+        if ((M).valid && a.do_sample && lane < 7) pmom = a.moments[(size_t)lane * a.ncells + (M).c]; \
+    } while (0)
+    DSMC_PREFETCH(m0);
+
+    while (m0.valid) {
+        const CellMeta m2 = load_meta(a, m0.c + 2 * G);
+        const uint32_t c = m0.c, s = m0.s, n = m0.n;
+        const bool active = m0.coeff > 0.0;
+        const bool in_lds = n <= K;
+        const double mom_old = pmom;
+        // current cell's velocities: registers -> LDS
+        if (active && in_lds) {
+#pragma unroll
+            for (int q = 0; q < PREFETCH_CHUNKS; q++) {
+                const uint32_t i = q * WAVE + lane;
+                if (i < n) { lx[i] = px[q]; ly[i] = py[q]; lz[i] = pz[q]; }
+            }
+            for (uint32_t i = PREFETCH_CHUNKS * WAVE + lane; i < n; i += WAVE) {
+                lx[i] = a.vx[s + i]; ly[i] = a.vy[s + i]; lz[i] = a.vz[s + i];
+            }
+        }
+        // next cell's velocities go in flight now
+        DSMC_PREFETCH(m1);
+        if (active) {
+            if (a.do_collide) {
+                const Member M = a.members[m0.member];
+                // Cell::prepare (cell.cpp:42-51)
+                const double select = m0.coeff * (double)n * (double)((int)n - 1) * m0.vrm + m0.rest;
+                const int pairs = (int)round(select);
+                const double rest = select - (double)pairs;
+                double crm = m0.vrm;
+                uint32_t acc = 0;
+                if (n >= 2) {
+                    if (in_lds) {
+                        __syncthreads();
+                        LdsStore S{lx, ly, lz, lmark};
+                        ntc_cell(M, c - M.cell0, step, n, pairs, m0.vrm, S, stage, crm, acc);
+                    } else {
+                        GlobalStore S{a.vx + s, a.vy + s, a.vz + s, a.scratch + s};
+                        ntc_cell(M, c - M.cell0, step, n, pairs, m0.vrm, S, stage, crm, acc);
+                    }
+                }
+                if (lane == 0) {
+                    a.rest[c] = rest;
+                    if (n >= 2) a.vr_max[c] = crm;
+                    a.stat_pairs[c] = pairs;
+                    a.stat_acc[c] = (int32_t)acc;
+                }
+                acc_total += acc;
+                cand_total += (n >= 2 && pairs > 0) ? (unsigned long long)pairs : 0ull;
+                overflow += in_lds ? 0ull : 1ull;
+            }
+            __syncthreads();
+            // per-cell FP64 moments and write-back
+            double mm[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (uint32_t i = lane; i < n; i += WAVE) {
+                float x, y, z;
+                if (in_lds) {
+                    x = lx[i]; y = ly[i]; z = lz[i];
+                    if (a.do_collide) { a.vx[s + i] = x; a.vy[s + i] = y; a.vz[s + i] = z; }
+                } else {
+                    x = GlobalStore::ldf(a.vx + s + i); y = GlobalStore::ldf(a.vy + s + i);
+                    z = GlobalStore::ldf(a.vz + s + i);
+                }
+                const double dx = x, dy = y, dz = z;
+                mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
+                mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+            }
+            if (a.do_sample) {
+#pragma unroll
+                for (int j = 0; j < 7; j++) mm[j] = wave_sum(mm[j]);
+                double mine = mm[0];
+#pragma unroll
+                for (int j = 1; j < 7; j++) mine = lane == (uint32_t)j ? mm[j] : mine;
+                if (lane < 7) a.moments[(size_t)lane * a.ncells + c] = mom_old + mine;
+            }
+            __syncthreads();
+        }
+        m0 = m1;
+        m1 = m2;
+    }
+#undef DSMC_PREFETCH
+    // acc/cand/overflow are wave-uniform (ballot counts)
+    if (lane == 0) {
+        if (acc_total) atomicAdd(&a.counters[CNT_COLL], acc_total);
+        if (cand_total) atomicAdd(&a.counters[CNT_CAND], cand_total);
+        if (overflow) atomicAdd(&a.counters[CNT_OVERFLOW], overflow);
     }
 }
 
@@ -597,10 +773,27 @@ hipError_t launch_bucket_cells(const BucketArgs &a, hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, hipStream_t s) {
+hipError_t launch_gather(const GatherArgs &a, uint32_t ncells, hipStream_t s) {
     if (ncells == 0) return hipSuccess;
-    size_t lds = (size_t)a.lds_cap * 16;
-    hipLaunchKernelGGL(k_gather_collide, dim3(ncells), dim3(64), lds, s, a);
+    hipLaunchKernelGGL(k_gather, dim3(ncells), dim3(64), 0, s, a);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, hipStream_t s) {
+    if (ncells == 0) return hipSuccess;
+    size_t lds = (size_t)a.lds_cap * 16 + 384 * 4;   // velocities, marks, pending-lane stage
+    // persistent waves: as many as fit on the device at once
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_collide, 64, lds);
+    if (e != hipSuccess) return e;
+    uint32_t grid = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
+    if (grid > ncells) grid = ncells;
+    hipLaunchKernelGGL(k_collide, dim3(grid), dim3(64), lds, s, a);
     LAUNCH_CHECK();
     return hipSuccess;
 }

@@ -1078,9 +1078,14 @@ static int collide_cell(or_system *s, int g) {
             ip0 = s->mol[g][ip0];
             ip1 = s->mol[g][ip1];
         } else {
+            /* one draw per candidate: word 0 picks the first particle and the
+             * low half of its product picks the second one's offset; word 1
+             * is the acceptance uniform, words 2-3 the scattering angles */
             philox_draw(s->p.key, TAG_SELECT, (uint32_t)s->cnt.steps, (uint32_t)g, (uint32_t)isel, o);
-            int a = (int)(((uint64_t)o[0] * (uint64_t)n) >> 32);
-            int b = (a + 1 + (int)(((uint64_t)o[1] * (uint64_t)(n - 1)) >> 32)) % n;
+            uint64_t prod = (uint64_t)o[0] * (uint64_t)n;
+            int a = (int)(prod >> 32);
+            uint32_t low = (uint32_t)prod;
+            int b = (a + 1 + (int)(((uint64_t)low * (uint64_t)(n - 1)) >> 32)) % n;
             ip0 = s->start[g] + a;
             ip1 = s->start[g] + b;
         }
@@ -1089,7 +1094,7 @@ static int collide_cell(or_system *s, int g) {
         if (v_rel > crm) crm = v_rel;
         s->cnt.candidates++;
         if (s->p.rng_mode == OR_RNG_RAN1) u_acc = or_ran1_next(&s->rnd);
-        else u_acc = or_u01(o[2]);
+        else u_acc = or_u01(o[1]);
         if (v_rel > u_acc * s->vr_max[g]) {
             collisions++;
             double uc, up;
@@ -1097,10 +1102,8 @@ static int collide_cell(or_system *s, int g) {
                 uc = or_ran1_next(&s->rnd);
                 up = or_ran1_next(&s->rnd);
             } else {
-                uint32_t q[4];
-                philox_draw(s->p.key, TAG_SCATTER, (uint32_t)s->cnt.steps, (uint32_t)g, (uint32_t)isel, q);
-                uc = or_u01(q[0]);
-                up = or_u01(q[1]);
+                uc = or_u01(o[2]);
+                up = or_u01(o[3]);
             }
             collide_pair(s, v0, v1, v_rel, uc, up);
         }
