@@ -11,23 +11,41 @@ timing starts.  One step = one System::step (accelerate, move with wall
 reflection, stable counting sort into cells, NTC collisions), per-cell FP64
 moment sampling included.
 
-Prints one JSON line (the driver contract) with a `roofline` object for the
-dominant kernel (algorithmic bytes: 56 B per particle = the 28-byte state
-read and written once) and a `cpu_baseline` object: the reference solver
-(oracle/_ref, compiled from the reference sources) when present, else the
-FP64 oracle, timed on this host's cores on a bounded sub-box sample.
+Prints one JSON line (the driver contract) with
+* `roofline`: the dominant kernel's algorithmic bytes per launch over its
+  mean launch time (HIP events on the library's stream), against the 8 TB/s
+  HBM peak.  Algorithmic bytes per particle: k_move 56 (28-byte state read
+  and written), k_gather 56 (state read from the old order, written in cell
+  order), k_collide 24 (velocities read and written); the step moves 140
+  (sort row 60 = gather 56 + the 4-byte cell key, read inside k_move).
+  `traffic` is the HBM bytes one launch of that kernel really moves, from two
+  rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over the same workload,
+  each scaled by what the counter reads for a plain 1 GiB copy in the same
+  process; null when rocprofv3 is unavailable.
+* `cpu_baseline`: the reference solver (oracle/_ref, compiled from the
+  reference sources) when present, else the FP64 oracle, timed on one host
+  core on a bounded sub-box sample of the same workload.
 """
 import argparse
+import glob
 import json
 import os
+import re
+import shutil
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STATE_BYTES = 28
+# algorithmic bytes per particle and launch
+KERNEL_BYTES = {"k_move": 56, "k_gather": 56, "k_collide": 24}
+STEP_BYTES = 140
+COPY_BYTES = 1 << 30
 
 
 def log(*a):
@@ -50,11 +68,50 @@ def workload(name):
     return p, w, ini
 
 
+def make_system(args):
+    from master_amd import dsmc
+    t_setup = time.time()
+    p, w, ini = workload(args.workload)
+    log(f"[bench] world {w.n} porosity {w.porosity:.4f} built in {time.time() - t_setup:.1f}s")
+    g = dsmc.System(ini, w)
+    n = g.initialize()
+    g.set_sampling(1)
+    if args.lds_cap:
+        g.set_option(dsmc.OPT_LDS_CAPACITY, args.lds_cap)
+    log(f"[bench] {n} particles seeded+sorted ({time.time() - t_setup:.1f}s), "
+        f"max cell occupancy {g.counters()['max_cell_occupancy']}")
+    return p, w, g, n
+
+
+class _StdoutToStderr:
+    """The reference solver prints progress on stdout; keep the bench's
+    stdout to the one JSON line by pointing fd 1 at stderr meanwhile."""
+
+    def __enter__(self):
+        import ctypes
+        self._libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self._saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        self._libc.fflush(None)
+        os.dup2(self._saved, 1)
+        os.close(self._saved)
+        return False
+
+
 def cpu_baseline(p, budget_s=20.0):
+    with _StdoutToStderr():
+        return _cpu_baseline(p, budget_s)
+
+
+def _cpu_baseline(p, budget_s):
     """Reference solver (or oracle) on a bounded sub-box of the same workload:
     the same geometry statistics, Kn and particles per pore voxel in a 64^3
     box, stepped until ~budget_s of CPU time.  Single core."""
-    import numpy as np
     from master_amd import config as C
     from oracle import binding as B
     n_sub = 64
@@ -86,6 +143,80 @@ def cpu_baseline(p, budget_s=20.0):
                       f"sphere radii), {steps} steps, {el:.1f} s CPU ({wall:.1f} s wall), FP64"}
 
 
+def pmc_pass(args):
+    """Body of one counter pass (run under rocprofv3 by measure_traffic):
+    a plain 1 GiB device copy three times (calibration), then the workload's
+    kernels for a few eager steps."""
+    import torch
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    torch.cuda.empty_cache()
+    _, _, g, n = make_system(args)
+    g.step(args.pmc_steps)
+    g.synchronize()
+    log(f"[pmc pass] {args.pmc_steps} steps done, {n} particles")
+
+
+def _short(name):
+    m = re.search(r"\b(k_\w+)\(", name)
+    return m.group(1) if m else name
+
+
+def _read_counter(db, counter, steps):
+    """Mean counter value (KiB) per dispatch: the last `steps` dispatches of
+    each library kernel, and the 1 GiB copy dispatches (by duration)."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, dispatch_id, duration, counter_value from pmc_events "
+                       "where counter_name = ? order by dispatch_id", (counter,)).fetchall()
+    con.close()
+    per, copy = {}, []
+    for name, _, dur, val in rows:
+        if "copyBuffer" in name and dur > 50_000:            # ns; the 1 GiB copies only
+            copy.append(val)
+        k = _short(name)
+        if k in KERNEL_BYTES:
+            per.setdefault(k, []).append(val)
+    out = {k: sum(v[-steps:]) / len(v[-steps:]) for k, v in per.items()}
+    return out, (sum(copy) / len(copy) if copy else None)
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of each hot kernel from two rocprofv3 counter
+    passes (a child process each), scaled by the same counter's reading of a
+    1 GiB copy.  Returns (per-kernel bytes, scale factors) or (None, reason)."""
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    base = tempfile.mkdtemp(prefix="dsmc_pmc_")
+    env = dict(os.environ, TMPDIR=base)
+    vals, scale = {}, {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = os.path.join(base, counter)
+        cmd = ["timeout", "-s", "KILL", str(args.pmc_timeout), prof, "--pmc", counter, "-d", d, "-o", "pass",
+               "--", sys.executable, os.path.abspath(__file__), "--pmc-pass", "--workload", args.workload,
+               "--lds-cap", str(args.lds_cap), "--pmc-steps", str(args.pmc_steps)]
+        log(f"[bench] counter pass {counter}")
+        r = subprocess.run(cmd, cwd=base, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        dbs = glob.glob(os.path.join(d, "**", "*.db"), recursive=True)
+        if r.returncode != 0 or not dbs:
+            log(r.stdout[-2000:])
+            return None, f"{counter} pass exited {r.returncode}"
+        per, copy_kib = _read_counter(dbs[0], counter, args.pmc_steps)
+        if not copy_kib:
+            return None, f"{counter}: calibration copy not found"
+        scale[counter] = COPY_BYTES / (copy_kib * 1024.0)
+        for k, v in per.items():
+            vals.setdefault(k, {})[counter] = v * 1024.0 * scale[counter]
+    shutil.rmtree(base, ignore_errors=True)
+    traffic = {k: v.get("FETCH_SIZE", 0.0) + v.get("WRITE_SIZE", 0.0) for k, v in vals.items()}
+    return {"bytes": traffic, "read": {k: v.get("FETCH_SIZE") for k, v in vals.items()},
+            "written": {k: v.get("WRITE_SIZE") for k, v in vals.items()}}, scale
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,38 +224,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="roofline", choices=["roofline", "precision", "cpu"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--lds-cap", type=int, default=0, help="particles per cell held in LDS (0: automatic)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="run only a plain 1 GiB device-to-device copy (rocprof counter calibration)")
+    ap.add_argument("--pmc-pass", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-steps", type=int, default=2, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-timeout", type=int, default=150, help=argparse.SUPPRESS)
     args = ap.parse_args()
-    if args.copy_calibration:
-        import torch
-        n = 1 << 28                                 # 2^28 float32 = 1 GiB
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
-        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the DSMC step here is a single-GPU path "
               "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_pass:
+        pmc_pass(args)
+        return
 
     from master_amd import dsmc
-    t_setup = time.time()
-    p, w, ini = workload(args.workload)
-    log(f"[bench] world {w.n} porosity {w.porosity:.4f} built in {time.time() - t_setup:.1f}s")
-    g = dsmc.System(ini, w)
-    n = g.initialize()
-    g.set_sampling(1)
-    if args.lds_cap:
-        g.set_option(dsmc.OPT_LDS_CAPACITY, args.lds_cap)
-    log(f"[bench] {n} particles seeded+sorted ({time.time() - t_setup:.1f}s), "
-        f"max cell occupancy {g.counters()['max_cell_occupancy']}")
+    p, w, g, n = make_system(args)
 
     # per-kernel device times (HIP events on the handle's stream), eager launches
     g.set_option(dsmc.OPT_TIMING, 1)
@@ -148,15 +265,35 @@ def main():
     ms = 1000 * el / args.steps
     log(f"[bench] {ms:.3f} ms/step, {value:.4g} particle-steps/s, collisions/step "
         f"{c['collisions'] / max(c['steps'], 1):.0f}, overflow cells {c['overflow_cells']}")
+    g.close()
 
-    dom = max(ktimes, key=ktimes.get)
-    achieved = 2 * STATE_BYTES * n / (ktimes[dom] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_particle": 2 * STATE_BYTES,
-            "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()},
-            "step_bytes_per_particle_pipeline": 4 * STATE_BYTES,
-            "step_frac": round(4 * STATE_BYTES * n / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
+    traffic, scale = (None, "skipped") if args.no_traffic else measure_traffic(args)
+    if traffic is None:
+        log(f"[bench] no traffic counters: {scale}")
+
+    per_kernel = {}
+    for k, t in ktimes.items():
+        e = {"ms": round(t, 4)}
+        if k in KERNEL_BYTES and t > 0:
+            gbs = KERNEL_BYTES[k] * n / (t * 1e-3) / 1e9
+            e.update(bytes_per_particle=KERNEL_BYTES[k], achieved_gbs=round(gbs, 1),
+                     frac=round(gbs / HBM_PEAK_GBS, 4))
+            if traffic and k in traffic["bytes"]:
+                e["traffic_bytes"] = round(traffic["bytes"][k])
+                e["traffic_per_particle"] = round(traffic["bytes"][k] / n, 2)
+        per_kernel[k] = e
+    dom = max((k for k in ktimes if k in KERNEL_BYTES), key=lambda k: ktimes[k])
+    achieved = per_kernel[dom]["achieved_gbs"]
+    step_gbs = STEP_BYTES * n / (el / args.steps) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": per_kernel[dom].get("traffic_bytes"),
+            "algorithmic_bytes": KERNEL_BYTES[dom] * n,
+            "launch_ms": per_kernel[dom]["ms"],
+            "traffic_scale": scale if traffic else None,
+            "kernels": per_kernel,
+            "step": {"bytes_per_particle": STEP_BYTES, "achieved_gbs": round(step_gbs, 1),
+                     "frac": round(step_gbs / HBM_PEAK_GBS, 4)}}
     cpu = None
     if not args.no_cpu_baseline:
         try:

@@ -48,12 +48,22 @@ __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        // one 32x32->64 multiply per word (v_mad_u64_u32), both halves used
+        const uint64_t p0 = (uint64_t)c0 * 0xD2511F53u;
+        const uint64_t p1 = (uint64_t)c2 * 0xCD9E8D57u;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
     }
     return make_uint4(c0, c1, c2, c3);
+}
+
+// Load through the scalar data cache (s_load): for tables no kernel writes
+// while it runs (member descriptors), at a wave-uniform address.  The
+// result lands in SGPRs and its wait (lgkmcnt) does not hold back the
+// vector loads in flight.
+template <class T>
+__device__ __forceinline__ T ld_const(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
 }
 
 // (2m+1) 2^-24 with m = x >> 9: exact in float and double, in (0, 1).

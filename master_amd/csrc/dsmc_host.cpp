@@ -174,6 +174,12 @@ dsmc_status build_member(dsmc_handle *h, const dsmc_settings &s, const dsmc_worl
         float inv = 1.0f / (float)b[a];
         if ((double)inv * b[a] < 1.0) inv = std::nextafter(inv, 2.0f);
         d.inv_b[a] = inv;
+        // the kernels' cell index (int)((float)i * inv) must be i / b for
+        // every voxel index of the axis: check them all
+        for (int i = 0; i < m.n[a]; i++) {
+            volatile float prod = (float)i * inv;
+            if ((int)prod != i / b[a]) return fail(h, DSMC_ERR_CONFIG, "cell index by reciprocal not exact");
+        }
     }
     d.s_wall = (float)std::sqrt(s.wall_temperature / s.mass);
     d.s_gas = (float)std::sqrt(s.temperature / s.mass);

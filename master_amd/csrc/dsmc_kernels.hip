@@ -409,7 +409,7 @@ struct GlobalStore {
 // front of the next batch; new candidates fill the remaining lanes.  The
 // result is the sequential NTC loop's, for every interleaving.
 template <class Store>
-__device__ __forceinline__ void ntc_cell(const Member &M, uint32_t cell_local, uint32_t step, uint32_t n,
+__device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint32_t cell_local, uint32_t step, uint32_t n,
                                          int pairs, double vrm, const Store &S, uint32_t *stage,
                                          double &crm_out, uint32_t &acc_out) {
     const uint32_t lane = threadIdx.x;
@@ -431,7 +431,7 @@ __device__ __forceinline__ void ntc_cell(const Member &M, uint32_t cell_local, u
         if (lane >= P) {
             const uint32_t kk = next + (lane - P);
             if (kk < npairs) {
-                const uint4 o = philox(TAG_SELECT, step, cell_local, kk, M.key_lo, M.key_hi);
+                const uint4 o = philox(TAG_SELECT, step, cell_local, kk, key_lo, key_hi);
                 k = kk;
                 i0 = __umulhi(o.x, n);
                 i1 = i0 + 1u + __umulhi(o.x * n, n - 1u);
@@ -526,32 +526,72 @@ extern __shared__ float dyn_lds[];
 struct CellMeta {
     uint32_t c, s, n;
     double coeff, vrm, rest;
-    uint32_t member;
+    uint32_t local, klo, khi;   // index inside the member, the member's Philox key
     bool valid;
 };
 
-__device__ __forceinline__ CellMeta load_meta(const CollideArgs &a, uint32_t c) {
+// Metadata of 64 consecutive cells of one wave's list, one cell per lane,
+// loaded with one vector load per field and read out with v_readlane: the
+// per-cell loop then issues no metadata load whose wait would also drain the
+// velocity prefetch (vmcnt counts in order).  vr_max and rest of a cell are
+// written only by the wave that owns it, after it has read them.
+struct MetaBatch {
+    uint32_t s, e, local, klo, khi;
+    double coeff, vrm, rest;
+};
+
+__device__ __forceinline__ MetaBatch load_batch(const CollideArgs &a, uint32_t j0) {
+    MetaBatch b{0u, 0u, 0u, 0u, 0u, 0.0, 0.0, 0.0};
+    const uint64_t c = (uint64_t)blockIdx.x + (uint64_t)(j0 + threadIdx.x) * gridDim.x;
+    if (c < a.ncells) {
+        b.s = a.start[c];
+        b.e = a.start[c + 1];
+        const Member &M = a.members[a.cell_member[c]];
+        b.local = (uint32_t)c - M.cell0;
+        b.klo = M.key_lo;
+        b.khi = M.key_hi;
+        b.coeff = a.coeff[c];
+        b.vrm = a.vr_max[c];
+        b.rest = a.rest[c];
+    }
+    return b;
+}
+
+__device__ __forceinline__ double readlane_d(double x, uint32_t l) {
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ CellMeta meta_at(const MetaBatch &b, uint32_t l, uint32_t c, uint32_t ncells) {
     CellMeta m;
     m.c = c;
-    m.valid = c < a.ncells;
-    m.s = 0; m.n = 0; m.coeff = 0.0; m.vrm = 0.0; m.rest = 0.0; m.member = 0;
-    if (m.valid) {
-        m.s = a.start[c];
-        m.n = a.start[c + 1] - m.s;
-        m.coeff = a.coeff[c];
-        m.vrm = a.vr_max[c];
-        m.rest = a.rest[c];
-        m.member = a.cell_member[c];
-    }
+    m.valid = c < ncells;
+    m.s = __builtin_amdgcn_readlane(b.s, l);
+    m.n = __builtin_amdgcn_readlane(b.e, l) - m.s;
+    m.local = __builtin_amdgcn_readlane(b.local, l);
+    m.klo = __builtin_amdgcn_readlane(b.klo, l);
+    m.khi = __builtin_amdgcn_readlane(b.khi, l);
+    m.coeff = readlane_d(b.coeff, l);
+    m.vrm = readlane_d(b.vrm, l);
+    m.rest = readlane_d(b.rest, l);
     return m;
 }
 
 static constexpr int PREFETCH_CHUNKS = 16;   // velocities of up to 1024 particles in registers
 
 // Persistent waves: wave w handles cells w, w + G, w + 2G, ... (G waves in
-// the grid).  While the NTC loop of one cell runs on LDS, the velocities of
-// the next cell are already in flight into registers and the metadata of
-// the one after is loading, so the loop's global-memory latency is hidden.
+// the grid).  Per cell, in this order:
+//   1. wait for the cell's velocities (prefetched one cell earlier) and put
+//      them into LDS;
+//   2. issue the loads of the next cell's velocities;
+//   3. issue the stores of the previous cell (its final velocities, kept in
+//      registers, its NTC state and its moments as fire-and-forget FP64
+//      atomics);
+//   4. NTC collisions and moments of this cell on LDS.
+// vmcnt counts loads and stores in issue order, so the only vector-memory
+// wait of the loop (step 1) covers operations issued a whole cell earlier.
 __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     const uint32_t lane = threadIdx.x;
     const uint32_t G = gridDim.x;
@@ -562,10 +602,17 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     const uint32_t step = *a.step;
     unsigned long long acc_total = 0, cand_total = 0, overflow = 0;
 
-    CellMeta m0 = load_meta(a, blockIdx.x);
-    CellMeta m1 = load_meta(a, blockIdx.x + G);
+    // cell j of this wave's list is blockIdx.x + j * G
+    MetaBatch cur = load_batch(a, 0), nxt = load_batch(a, WAVE);
+    uint32_t j = 0;
+    CellMeta m0 = meta_at(cur, 0, blockIdx.x, a.ncells);
     float px[PREFETCH_CHUNKS], py[PREFETCH_CHUNKS], pz[PREFETCH_CHUNKS];
-    double pmom = 0.0;
+    float wx[PREFETCH_CHUNKS], wy[PREFETCH_CHUNKS], wz[PREFETCH_CHUNKS];
+    // the previous cell's outstanding writes
+    bool w_any = false, w_back = false, w_coll = false;
+    uint32_t w_c = 0, w_s = 0, w_n = 0, w_acc = 0;
+    int w_pairs = 0;
+    double w_rest = 0.0, w_crm = 0.0, w_mom = 0.0;
 #define DSMC_PREFETCH(M)                                                                    \
     do {                                                                                    \
         _Pragma("unroll") for (int q = 0; q < PREFETCH_CHUNKS; q++) {                       \
@@ -574,18 +621,43 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
                 px[q] = a.vx[(M).s + i]; py[q] = a.vy[(M).s + i]; pz[q] = a.vz[(M).s + i];  \
             }                                                                               \
         }                                                                                   \
-        pmom = 0.0;                                                                         \
-        if ((M).valid && a.do_sample && lane < 7) pmom = a.moments[(size_t)lane * a.ncells + (M).c]; \
+    } while (0)
+#define DSMC_FLUSH()                                                                        \
+    do {                                                                                    \
+        if (w_any) {                                                                        \
+            if (w_back) {                                                                   \
+                _Pragma("unroll") for (int q = 0; q < PREFETCH_CHUNKS; q++) {               \
+                    const uint32_t i = q * WAVE + lane;                                     \
+                    if (i < w_n) {                                                          \
+                        a.vx[w_s + i] = wx[q]; a.vy[w_s + i] = wy[q]; a.vz[w_s + i] = wz[q]; \
+                    }                                                                       \
+                }                                                                           \
+            }                                                                               \
+            if (w_coll && lane == 0) {                                                      \
+                a.rest[w_c] = w_rest;                                                       \
+                if (w_n >= 2) a.vr_max[w_c] = w_crm;                                        \
+                a.stat_pairs[w_c] = w_pairs;                                                \
+                a.stat_acc[w_c] = (int32_t)w_acc;                                           \
+            }                                                                               \
+            if (a.do_sample && lane < 7)                                                    \
+                unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + w_c, w_mom);         \
+        }                                                                                   \
+        w_any = false;                                                                      \
     } while (0)
     DSMC_PREFETCH(m0);
 
     while (m0.valid) {
-        const CellMeta m2 = load_meta(a, m0.c + 2 * G);
+        // the next cell's metadata; a new batch 64 cells ahead when this one ends
+        const uint32_t j1 = j + 1;
+        if ((j1 & (WAVE - 1)) == 0) {
+            cur = nxt;
+            nxt = load_batch(a, j1 + WAVE);
+        }
+        const CellMeta m1 = meta_at(cur, j1 & (WAVE - 1), blockIdx.x + j1 * G, a.ncells);
         const uint32_t c = m0.c, s = m0.s, n = m0.n;
         const bool active = m0.coeff > 0.0;
         const bool in_lds = n <= K;
-        const double mom_old = pmom;
-        // current cell's velocities: registers -> LDS
+        // 1. this cell's velocities: registers -> LDS
         if (active && in_lds) {
 #pragma unroll
             for (int q = 0; q < PREFETCH_CHUNKS; q++) {
@@ -596,67 +668,90 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
                 lx[i] = a.vx[s + i]; ly[i] = a.vy[s + i]; lz[i] = a.vz[s + i];
             }
         }
-        // next cell's velocities go in flight now
+        // 2. the next cell's velocities go in flight
         DSMC_PREFETCH(m1);
+        // 3. the previous cell's writes
+        DSMC_FLUSH();
         if (active) {
+            // 4. NTC (Cell::prepare cell.cpp:42-51, Cell::collide 75-111)
             if (a.do_collide) {
-                const Member M = a.members[m0.member];
-                // Cell::prepare (cell.cpp:42-51)
                 const double select = m0.coeff * (double)n * (double)((int)n - 1) * m0.vrm + m0.rest;
                 const int pairs = (int)round(select);
-                const double rest = select - (double)pairs;
                 double crm = m0.vrm;
                 uint32_t acc = 0;
                 if (n >= 2) {
                     if (in_lds) {
                         __syncthreads();
                         LdsStore S{lx, ly, lz, lmark};
-                        ntc_cell(M, c - M.cell0, step, n, pairs, m0.vrm, S, stage, crm, acc);
+                        ntc_cell(m0.klo, m0.khi, m0.local, step, n, pairs, m0.vrm, S, stage, crm, acc);
                     } else {
                         GlobalStore S{a.vx + s, a.vy + s, a.vz + s, a.scratch + s};
-                        ntc_cell(M, c - M.cell0, step, n, pairs, m0.vrm, S, stage, crm, acc);
+                        ntc_cell(m0.klo, m0.khi, m0.local, step, n, pairs, m0.vrm, S, stage, crm, acc);
                     }
                 }
-                if (lane == 0) {
-                    a.rest[c] = rest;
-                    if (n >= 2) a.vr_max[c] = crm;
-                    a.stat_pairs[c] = pairs;
-                    a.stat_acc[c] = (int32_t)acc;
-                }
+                w_rest = select - (double)pairs;
+                w_crm = crm;
+                w_pairs = pairs;
+                w_acc = acc;
                 acc_total += acc;
                 cand_total += (n >= 2 && pairs > 0) ? (unsigned long long)pairs : 0ull;
                 overflow += in_lds ? 0ull : 1ull;
             }
             __syncthreads();
-            // per-cell FP64 moments and write-back
+            // per-cell FP64 moments; final velocities to registers for step 3.
+            // The two paths stay apart: a merged loop would make the LDS reads
+            // wait on the other path's global loads, and so on every load in
+            // flight (vmcnt drains in order).
             double mm[7] = {0, 0, 0, 0, 0, 0, 0};
-            for (uint32_t i = lane; i < n; i += WAVE) {
-                float x, y, z;
-                if (in_lds) {
-                    x = lx[i]; y = ly[i]; z = lz[i];
-                    if (a.do_collide) { a.vx[s + i] = x; a.vy[s + i] = y; a.vz[s + i] = z; }
-                } else {
-                    x = GlobalStore::ldf(a.vx + s + i); y = GlobalStore::ldf(a.vy + s + i);
-                    z = GlobalStore::ldf(a.vz + s + i);
+            if (in_lds) {
+#pragma unroll
+                for (int q = 0; q < PREFETCH_CHUNKS; q++) {
+                    const uint32_t i = q * WAVE + lane;
+                    float x = 0.0f, y = 0.0f, z = 0.0f;
+                    if (i < n) {
+                        x = lx[i]; y = ly[i]; z = lz[i];
+                        const double dx = x, dy = y, dz = z;
+                        mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
+                        mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+                    }
+                    wx[q] = x; wy[q] = y; wz[q] = z;
                 }
-                const double dx = x, dy = y, dz = z;
-                mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
-                mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+                for (uint32_t i = PREFETCH_CHUNKS * WAVE + lane; i < n; i += WAVE) {
+                    const float x = lx[i], y = ly[i], z = lz[i];
+                    if (a.do_collide) { a.vx[s + i] = x; a.vy[s + i] = y; a.vz[s + i] = z; }
+                    const double dx = x, dy = y, dz = z;
+                    mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
+                    mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+                }
+            } else {
+                for (uint32_t i = lane; i < n; i += WAVE) {
+                    const double dx = GlobalStore::ldf(a.vx + s + i), dy = GlobalStore::ldf(a.vy + s + i),
+                                 dz = GlobalStore::ldf(a.vz + s + i);
+                    mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
+                    mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+                }
             }
+            w_mom = 0.0;
             if (a.do_sample) {
 #pragma unroll
-                for (int j = 0; j < 7; j++) mm[j] = wave_sum(mm[j]);
+                for (int k = 0; k < 7; k++) mm[k] = wave_sum(mm[k]);
                 double mine = mm[0];
 #pragma unroll
-                for (int j = 1; j < 7; j++) mine = lane == (uint32_t)j ? mm[j] : mine;
-                if (lane < 7) a.moments[(size_t)lane * a.ncells + c] = mom_old + mine;
+                for (int k = 1; k < 7; k++) mine = lane == (uint32_t)k ? mm[k] : mine;
+                w_mom = mine;
             }
+            w_any = true;
+            w_back = a.do_collide && in_lds;
+            w_coll = a.do_collide;
+            w_c = c; w_s = s; w_n = n;
             __syncthreads();
         }
         m0 = m1;
-        m1 = m2;
+        j = j1;
     }
+    DSMC_FLUSH();
 #undef DSMC_PREFETCH
+#undef DSMC_FLUSH
     // acc/cand/overflow are wave-uniform (ballot counts)
     if (lane == 0) {
         if (acc_total) atomicAdd(&a.counters[CNT_COLL], acc_total);
