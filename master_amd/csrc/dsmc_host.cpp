@@ -417,7 +417,6 @@ CollideArgs collide_args(dsmc_handle *h, bool collide, bool sample) {
     c.stat_pairs = h->d_pairs;
     c.stat_acc = h->d_acc;
     c.moments = h->d_mom;
-    c.scratch = h->d_mv_cell;
     c.step = h->d_step;
     c.counters = h->d_cnt;
     c.ncells = h->ncells;

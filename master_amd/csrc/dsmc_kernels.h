@@ -82,7 +82,6 @@ struct CollideArgs {
     double *vr_max, *rest;
     int32_t *stat_pairs, *stat_acc;
     double *moments;                    // 7 x ncells (SoA)
-    uint32_t *scratch;                  // per-particle marks for cells beyond lds_cap
     const uint32_t *step;
     unsigned long long *counters;
     uint32_t ncells, cell_begin, lds_cap;

@@ -362,24 +362,19 @@ __global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
 }
 
 // ------------------------------------------------------------------ collide
-// Velocity and mark storage of one cell: LDS when the cell fits, else the
-// cell's own range of the state arrays plus a per-particle scratch array in
-// global memory (accessed coherently, the wave owns the range).
+// Velocities of one cell: LDS when the cell fits, else the cell's own range
+// of the state arrays in global memory (accessed coherently, the wave owns
+// the range).
 struct LdsStore {
     float *x, *y, *z;
-    uint32_t *mark;
     __device__ __forceinline__ float lx(uint32_t i) const { return x[i]; }
     __device__ __forceinline__ float ly(uint32_t i) const { return y[i]; }
     __device__ __forceinline__ float lz(uint32_t i) const { return z[i]; }
     __device__ __forceinline__ void st(uint32_t i, float a, float b, float c) const { x[i] = a; y[i] = b; z[i] = c; }
-    __device__ __forceinline__ void mark_min(uint32_t i, uint32_t v) const { atomicMin(&mark[i], v); }
-    __device__ __forceinline__ uint32_t mark_ld(uint32_t i) const { return mark[i]; }
-    __device__ __forceinline__ void mark_st(uint32_t i, uint32_t v) const { mark[i] = v; }
 };
 
 struct GlobalStore {
     float *x, *y, *z;
-    uint32_t *mark;
     __device__ __forceinline__ static float ldf(float *p) {
         return __uint_as_float(__hip_atomic_load((uint32_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
@@ -392,44 +387,45 @@ struct GlobalStore {
     __device__ __forceinline__ void st(uint32_t i, float a, float b, float c) const {
         stf(x + i, a); stf(y + i, b); stf(z + i, c);
     }
-    __device__ __forceinline__ void mark_min(uint32_t i, uint32_t v) const { atomicMin(&mark[i], v); }
-    __device__ __forceinline__ uint32_t mark_ld(uint32_t i) const {
-        return __hip_atomic_load(mark + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __device__ __forceinline__ void mark_st(uint32_t i, uint32_t v) const {
-        __hip_atomic_store(mark + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
 };
 
 // Cell::collide (cell.cpp:75-111) for one cell, candidates in their
-// sequential order.  A wave evaluates up to 64 candidates at once; a
-// candidate commits unless an earlier candidate of the batch that touches
-// one of its particles either collides now or is itself still pending (its
-// outcome unknown).  Pending candidates keep their draws and move to the
-// front of the next batch; new candidates fill the remaining lanes.  The
-// result is the sequential NTC loop's, for every interleaving.
+// sequential order.  A wave holds up to 64 candidates at once; a candidate
+// commits unless an earlier candidate (smaller index) held by the wave that
+// shares a particle with it either collides now or is itself still pending
+// (its outcome unknown).  Pending candidates keep their lanes and draws and
+// are evaluated again in the next round; the other lanes take the next
+// candidates.  The result is the sequential NTC loop's, for every
+// interleaving.
+//
+// Conflicts go through a table of MARKS words in LDS indexed by particle
+// mod MARKS, holding the smallest candidate index of a blocker touching such
+// a particle (ds_min).  Two particles sharing a word only make a candidate
+// wait a round without need; they never let a dependent one through.
+static constexpr uint32_t MARKS = 256;
+
 template <class Store>
-__device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint32_t cell_local, uint32_t step, uint32_t n,
-                                         int pairs, double vrm, const Store &S, uint32_t *stage,
+__device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint32_t cell_local, uint32_t step,
+                                         uint32_t n, int pairs, double vrm, const Store &S, uint32_t *mark,
                                          double &crm_out, uint32_t &acc_out) {
     const uint32_t lane = threadIdx.x;
     const uint32_t NONE = 0xffffffffu;
-    for (uint32_t i = lane; i < n; i += WAVE) S.mark_st(i, NONE);
-    __syncthreads();
     double crm = vrm;
     uint32_t acc = 0;
     const uint32_t npairs = pairs > 0 ? (uint32_t)pairs : 0u;
     uint32_t next = 0;       // next candidate index not yet in a lane
-    uint32_t P = 0;          // pending candidates held in lanes [0, P)
     uint32_t k = NONE, i0 = 0, i1 = 0, zc = 0, zp = 0;
     float u = 0.0f;
+    bool pending = false;
     for (;;) {
-        // fill lanes [P, 64) with new candidates, in order.  One Philox call
-        // per candidate: word 0 picks the first particle (its product's low
-        // half, uniform and independent of the pick, the offset of the
-        // second), word 1 the acceptance, words 2-3 the scattering angles.
-        if (lane >= P) {
-            const uint32_t kk = next + (lane - P);
+        // lanes not holding a pending candidate take the next candidates, in
+        // lane order.  One Philox call per candidate: word 0 picks the first
+        // particle (its product's low half, uniform and independent of the
+        // pick, the offset of the second), word 1 the acceptance, words 2-3
+        // the scattering angles.
+        const uint64_t fm = __ballot(!pending);
+        if (!pending) {
+            const uint32_t kk = next + mbcnt(fm);
             if (kk < npairs) {
                 const uint4 o = philox(TAG_SELECT, step, cell_local, kk, key_lo, key_hi);
                 k = kk;
@@ -444,11 +440,12 @@ __device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint3
             }
         }
         {
-            const uint32_t room = (uint32_t)WAVE - P;
+            const uint32_t room = (uint32_t)__popcll(fm);
             next = (npairs - next < room) ? npairs : next + room;
         }
         const bool valid = k != NONE;
         if (__ballot(valid) == 0ull) break;
+        const uint32_t h0 = i0 & (MARKS - 1), h1 = i1 & (MARKS - 1);
         float vrel = 0.0f;
         bool accept = false;
         if (valid) {
@@ -459,17 +456,17 @@ __device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint3
             accept = (double)vrel > (double)u * vrm;
         }
         // blockers: accepted candidates, then (closure) pending ones
-        bool blocker = accept, pending = false, marked = accept;
-        if (accept) { S.mark_min(i0, lane); S.mark_min(i1, lane); }
+        bool marked = accept;
+        pending = false;
+        if (accept) { atomicMin(&mark[h0], k); atomicMin(&mark[h1], k); }
         for (;;) {
             __syncthreads();
-            const bool dep = valid && (S.mark_ld(i0) < lane || S.mark_ld(i1) < lane);
+            const bool dep = valid && (mark[h0] < k || mark[h1] < k);
             pending = pending || dep;
             const bool add = pending && !marked;
             if (__ballot(add) == 0ull) break;
-            if (add) { S.mark_min(i0, lane); S.mark_min(i1, lane); marked = true; }
+            if (add) { atomicMin(&mark[h0], k); atomicMin(&mark[h1], k); marked = true; }
         }
-        (void)blocker;
         const bool commit = valid && !pending;
         if (commit && (double)vrel > crm) crm = (double)vrel;
         const bool scatter = commit && accept;
@@ -491,29 +488,7 @@ __device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint3
             S.st(i0, cmx + 0.5f * vrx, cmy + 0.5f * vry, cmz + 0.5f * vrz);
             S.st(i1, cmx - 0.5f * vrx, cmy - 0.5f * vry, cmz - 0.5f * vrz);
         }
-        if (marked) { S.mark_st(i0, NONE); S.mark_st(i1, NONE); }
-        // pending candidates to the front, in order
-        const uint64_t pm = __ballot(pending);
-        const uint32_t np = (uint32_t)__popcll(pm);
-        if (pending) {
-            const uint32_t r = mbcnt(pm);
-            stage[r] = k;
-            stage[64 + r] = i0;
-            stage[128 + r] = i1;
-            stage[192 + r] = __float_as_uint(u);
-            stage[256 + r] = zc;
-            stage[320 + r] = zp;
-        }
-        __syncthreads();
-        if (lane < np) {
-            k = stage[lane];
-            i0 = stage[64 + lane];
-            i1 = stage[128 + lane];
-            u = __uint_as_float(stage[192 + lane]);
-            zc = stage[256 + lane];
-            zp = stage[320 + lane];
-        }
-        P = np;
+        if (marked) { mark[h0] = NONE; mark[h1] = NONE; }
         __syncthreads();
     }
     crm_out = wave_max(crm);
@@ -582,23 +557,20 @@ __device__ __forceinline__ CellMeta meta_at(const MetaBatch &b, uint32_t l, uint
 static constexpr int PREFETCH_CHUNKS = 16;   // velocities of up to 1024 particles in registers
 
 // Persistent waves: wave w handles cells w, w + G, w + 2G, ... (G waves in
-// the grid).  Per cell, in this order:
-//   1. wait for the cell's velocities (prefetched one cell earlier) and put
-//      them into LDS;
-//   2. issue the loads of the next cell's velocities;
-//   3. issue the stores of the previous cell (its final velocities, kept in
-//      registers, its NTC state and its moments as fire-and-forget FP64
-//      atomics);
-//   4. NTC collisions and moments of this cell on LDS.
-// vmcnt counts loads and stores in issue order, so the only vector-memory
-// wait of the loop (step 1) covers operations issued a whole cell earlier.
+// the grid).  Per cell: wait for its velocities (prefetched one cell
+// earlier) and put them into LDS; issue the loads of the next cell's
+// velocities; NTC collisions on LDS; write-back, FP64 moments (as
+// fire-and-forget atomics) and the cell's NTC state.  The loop's one
+// vector-memory wait therefore covers loads issued a whole cell earlier and
+// the previous cell's stores.  LDS: 12 bytes per particle slot and the
+// 1 KiB mark table.
 __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     const uint32_t lane = threadIdx.x;
     const uint32_t G = gridDim.x;
     const uint32_t K = a.lds_cap;
     float *lx = dyn_lds, *ly = dyn_lds + K, *lz = dyn_lds + 2 * K;
-    uint32_t *lmark = (uint32_t *)(dyn_lds + 3 * K);
-    uint32_t *stage = lmark + K;
+    uint32_t *mark = (uint32_t *)(dyn_lds + 3 * K);
+    for (uint32_t i = lane; i < MARKS; i += WAVE) mark[i] = 0xffffffffu;
     const uint32_t step = *a.step;
     unsigned long long acc_total = 0, cand_total = 0, overflow = 0;
 
@@ -607,12 +579,6 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     uint32_t j = 0;
     CellMeta m0 = meta_at(cur, 0, blockIdx.x, a.ncells);
     float px[PREFETCH_CHUNKS], py[PREFETCH_CHUNKS], pz[PREFETCH_CHUNKS];
-    float wx[PREFETCH_CHUNKS], wy[PREFETCH_CHUNKS], wz[PREFETCH_CHUNKS];
-    // the previous cell's outstanding writes
-    bool w_any = false, w_back = false, w_coll = false;
-    uint32_t w_c = 0, w_s = 0, w_n = 0, w_acc = 0;
-    int w_pairs = 0;
-    double w_rest = 0.0, w_crm = 0.0, w_mom = 0.0;
 #define DSMC_PREFETCH(M)                                                                    \
     do {                                                                                    \
         _Pragma("unroll") for (int q = 0; q < PREFETCH_CHUNKS; q++) {                       \
@@ -621,28 +587,6 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
                 px[q] = a.vx[(M).s + i]; py[q] = a.vy[(M).s + i]; pz[q] = a.vz[(M).s + i];  \
             }                                                                               \
         }                                                                                   \
-    } while (0)
-#define DSMC_FLUSH()                                                                        \
-    do {                                                                                    \
-        if (w_any) {                                                                        \
-            if (w_back) {                                                                   \
-                _Pragma("unroll") for (int q = 0; q < PREFETCH_CHUNKS; q++) {               \
-                    const uint32_t i = q * WAVE + lane;                                     \
-                    if (i < w_n) {                                                          \
-                        a.vx[w_s + i] = wx[q]; a.vy[w_s + i] = wy[q]; a.vz[w_s + i] = wz[q]; \
-                    }                                                                       \
-                }                                                                           \
-            }                                                                               \
-            if (w_coll && lane == 0) {                                                      \
-                a.rest[w_c] = w_rest;                                                       \
-                if (w_n >= 2) a.vr_max[w_c] = w_crm;                                        \
-                a.stat_pairs[w_c] = w_pairs;                                                \
-                a.stat_acc[w_c] = (int32_t)w_acc;                                           \
-            }                                                                               \
-            if (a.do_sample && lane < 7)                                                    \
-                unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + w_c, w_mom);         \
-        }                                                                                   \
-        w_any = false;                                                                      \
     } while (0)
     DSMC_PREFETCH(m0);
 
@@ -657,7 +601,7 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
         const uint32_t c = m0.c, s = m0.s, n = m0.n;
         const bool active = m0.coeff > 0.0;
         const bool in_lds = n <= K;
-        // 1. this cell's velocities: registers -> LDS
+        // this cell's velocities: registers -> LDS
         if (active && in_lds) {
 #pragma unroll
             for (int q = 0; q < PREFETCH_CHUNKS; q++) {
@@ -668,12 +612,10 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
                 lx[i] = a.vx[s + i]; ly[i] = a.vy[s + i]; lz[i] = a.vz[s + i];
             }
         }
-        // 2. the next cell's velocities go in flight
+        // the next cell's velocities go in flight
         DSMC_PREFETCH(m1);
-        // 3. the previous cell's writes
-        DSMC_FLUSH();
         if (active) {
-            // 4. NTC (Cell::prepare cell.cpp:42-51, Cell::collide 75-111)
+            // Cell::prepare (cell.cpp:42-51) and Cell::collide (75-111)
             if (a.do_collide) {
                 const double select = m0.coeff * (double)n * (double)((int)n - 1) * m0.vrm + m0.rest;
                 const int pairs = (int)round(select);
@@ -681,42 +623,29 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
                 uint32_t acc = 0;
                 if (n >= 2) {
                     if (in_lds) {
-                        __syncthreads();
-                        LdsStore S{lx, ly, lz, lmark};
-                        ntc_cell(m0.klo, m0.khi, m0.local, step, n, pairs, m0.vrm, S, stage, crm, acc);
+                        LdsStore S{lx, ly, lz};
+                        ntc_cell(m0.klo, m0.khi, m0.local, step, n, pairs, m0.vrm, S, mark, crm, acc);
                     } else {
-                        GlobalStore S{a.vx + s, a.vy + s, a.vz + s, a.scratch + s};
-                        ntc_cell(m0.klo, m0.khi, m0.local, step, n, pairs, m0.vrm, S, stage, crm, acc);
+                        GlobalStore S{a.vx + s, a.vy + s, a.vz + s};
+                        ntc_cell(m0.klo, m0.khi, m0.local, step, n, pairs, m0.vrm, S, mark, crm, acc);
                     }
                 }
-                w_rest = select - (double)pairs;
-                w_crm = crm;
-                w_pairs = pairs;
-                w_acc = acc;
+                if (lane == 0) {
+                    a.rest[c] = select - (double)pairs;
+                    if (n >= 2) a.vr_max[c] = crm;
+                    a.stat_pairs[c] = pairs;
+                    a.stat_acc[c] = (int32_t)acc;
+                }
                 acc_total += acc;
                 cand_total += (n >= 2 && pairs > 0) ? (unsigned long long)pairs : 0ull;
                 overflow += in_lds ? 0ull : 1ull;
             }
-            __syncthreads();
-            // per-cell FP64 moments; final velocities to registers for step 3.
-            // The two paths stay apart: a merged loop would make the LDS reads
-            // wait on the other path's global loads, and so on every load in
-            // flight (vmcnt drains in order).
+            // write-back and per-cell FP64 moments.  The two paths stay
+            // apart: a merged loop would make the LDS reads wait on the other
+            // path's global loads, and so on every load in flight.
             double mm[7] = {0, 0, 0, 0, 0, 0, 0};
             if (in_lds) {
-#pragma unroll
-                for (int q = 0; q < PREFETCH_CHUNKS; q++) {
-                    const uint32_t i = q * WAVE + lane;
-                    float x = 0.0f, y = 0.0f, z = 0.0f;
-                    if (i < n) {
-                        x = lx[i]; y = ly[i]; z = lz[i];
-                        const double dx = x, dy = y, dz = z;
-                        mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
-                        mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
-                    }
-                    wx[q] = x; wy[q] = y; wz[q] = z;
-                }
-                for (uint32_t i = PREFETCH_CHUNKS * WAVE + lane; i < n; i += WAVE) {
+                for (uint32_t i = lane; i < n; i += WAVE) {
                     const float x = lx[i], y = ly[i], z = lz[i];
                     if (a.do_collide) { a.vx[s + i] = x; a.vy[s + i] = y; a.vz[s + i] = z; }
                     const double dx = x, dy = y, dz = z;
@@ -731,27 +660,19 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
                     mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
                 }
             }
-            w_mom = 0.0;
             if (a.do_sample) {
 #pragma unroll
-                for (int k = 0; k < 7; k++) mm[k] = wave_sum(mm[k]);
+                for (int q = 0; q < 7; q++) mm[q] = wave_sum(mm[q]);
                 double mine = mm[0];
 #pragma unroll
-                for (int k = 1; k < 7; k++) mine = lane == (uint32_t)k ? mm[k] : mine;
-                w_mom = mine;
+                for (int q = 1; q < 7; q++) mine = lane == (uint32_t)q ? mm[q] : mine;
+                if (lane < 7) unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + c, mine);
             }
-            w_any = true;
-            w_back = a.do_collide && in_lds;
-            w_coll = a.do_collide;
-            w_c = c; w_s = s; w_n = n;
-            __syncthreads();
         }
         m0 = m1;
         j = j1;
     }
-    DSMC_FLUSH();
 #undef DSMC_PREFETCH
-#undef DSMC_FLUSH
     // acc/cand/overflow are wave-uniform (ballot counts)
     if (lane == 0) {
         if (acc_total) atomicAdd(&a.counters[CNT_COLL], acc_total);
@@ -877,7 +798,7 @@ hipError_t launch_gather(const GatherArgs &a, uint32_t ncells, hipStream_t s) {
 
 hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, hipStream_t s) {
     if (ncells == 0) return hipSuccess;
-    size_t lds = (size_t)a.lds_cap * 16 + 384 * 4;   // velocities, marks, pending-lane stage
+    size_t lds = (size_t)a.lds_cap * 12 + MARKS * 4;  // one cell's velocities, mark table
     // persistent waves: as many as fit on the device at once
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
