@@ -80,6 +80,7 @@ struct dsmc_handle {
     int64_t nsamples = 0;
     bool initialized = false;
     int64_t max_occ = 0;
+    unsigned long long overflow_seen = 0;   // overflow count at the last LDS-capacity refresh
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     hipGraph_t graph_src[2] = {nullptr, nullptr};
     bool graph_sample[2] = {false, false};
@@ -256,7 +257,7 @@ dsmc_status upload_geometry(dsmc_handle *h) {
     HIPCHK(h, dalloc(&h->d_bfill, cells));
     HIPCHK(h, dalloc(&h->d_bsums, h->nblocks));
     HIPCHK(h, dalloc(&h->d_step, 1));
-    HIPCHK(h, dalloc(&h->d_mvcount, 1));
+    HIPCHK(h, dalloc(&h->d_mvcount, 2));
     HIPCHK(h, dalloc(&h->d_cnt, CNT_NUM));
     HIPCHK(h, dalloc(&h->d_partial, 2 * TOT_BLOCKS * 4));
     HIPCHK(h, hipMemcpy(h->d_members, members.data(), members.size() * sizeof(Member), hipMemcpyHostToDevice));
@@ -347,9 +348,10 @@ struct Scope {
 
 uint32_t auto_lds_cap(int64_t max_occ) {
     // LDS per cell bounds the collide kernel's occupancy: a small margin over
-    // the largest cell; cells that outgrow it take the global-memory path
-    int64_t k = max_occ + 64;
-    k = (k + 63) / 64 * 64;
+    // the largest cell, rounded to 32 slots; cells that outgrow it take the
+    // global-memory path until the next refresh
+    int64_t k = max_occ + 16;
+    k = (k + 31) / 32 * 32;
     if (k < 128) k = 128;
     if (k > 4096) k = 4096;
     return (uint32_t)k;
@@ -419,6 +421,7 @@ CollideArgs collide_args(dsmc_handle *h, bool collide, bool sample) {
     c.moments = h->d_mom;
     c.step = h->d_step;
     c.counters = h->d_cnt;
+    c.queue = h->d_mvcount + 1;
     c.ncells = h->ncells;
     c.cell_begin = 0;
     c.lds_cap = h->lds_cap;
@@ -869,8 +872,15 @@ dsmc_status dsmc_get_counters(dsmc_handle *h, dsmc_counters *c) {
     c->errors = v[CNT_ERROR];
     c->candidates = v[CNT_CAND];
     c->num_molecules = h->N;
-    c->max_cell_occupancy = h->max_occ;
     c->overflow_cells = v[CNT_OVERFLOW];
+    // cells outgrew the collide kernel's LDS slots since the last refresh:
+    // size them again (the caller has synchronised here anyway)
+    if (h->lds_auto && v[CNT_OVERFLOW] > h->overflow_seen) {
+        h->overflow_seen = v[CNT_OVERFLOW];
+        dsmc_status st = refresh_occupancy(h);
+        if (st != DSMC_OK) return st;
+    }
+    c->max_cell_occupancy = h->max_occ;
     return DSMC_OK;
 }
 

@@ -84,7 +84,9 @@ struct CollideArgs {
     double *moments;                    // 7 x ncells (SoA)
     const uint32_t *step;
     unsigned long long *counters;
+    uint32_t *queue;                    // work queue counter, zeroed by k_begin_step
     uint32_t ncells, cell_begin, lds_cap;
+    uint32_t chunk;                     // cells per queue entry (set by launch_collide)
     int do_collide, do_sample;
 };
 

@@ -447,11 +447,12 @@ __device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint3
         if (__ballot(valid) == 0ull) break;
         const uint32_t h0 = i0 & (MARKS - 1), h1 = i1 & (MARKS - 1);
         float vrel = 0.0f;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
         bool accept = false;
         if (valid) {
-            const float dx = S.lx(i0) - S.lx(i1);
-            const float dy = S.ly(i0) - S.ly(i1);
-            const float dz = S.lz(i0) - S.lz(i1);
+            ax = S.lx(i0); ay = S.ly(i0); az = S.lz(i0);
+            bx = S.lx(i1); by = S.ly(i1); bz = S.lz(i1);
+            const float dx = ax - bx, dy = ay - by, dz = az - bz;
             vrel = sqrtf(dx * dx + dy * dy + dz * dz);
             accept = (double)vrel > (double)u * vrm;
         }
@@ -482,8 +483,9 @@ __device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint3
             const float vrx = vrel * cos_th;
             const float vry = vrel * sin_th * cp;
             const float vrz = vrel * sin_th * sp;
-            const float ax = S.lx(i0), ay = S.ly(i0), az = S.lz(i0);
-            const float bx = S.lx(i1), by = S.ly(i1), bz = S.lz(i1);
+            // a committed candidate's particles are touched by no other
+            // committed collision of this round: the velocities read above
+            // are still current
             const float cmx = 0.5f * (ax + bx), cmy = 0.5f * (ay + by), cmz = 0.5f * (az + bz);
             S.st(i0, cmx + 0.5f * vrx, cmy + 0.5f * vry, cmz + 0.5f * vrz);
             S.st(i1, cmx - 0.5f * vrx, cmy - 0.5f * vry, cmz - 0.5f * vrz);
@@ -505,20 +507,29 @@ struct CellMeta {
     bool valid;
 };
 
-// Metadata of 64 consecutive cells of one wave's list, one cell per lane,
-// loaded with one vector load per field and read out with v_readlane: the
-// per-cell loop then issues no metadata load whose wait would also drain the
-// velocity prefetch (vmcnt counts in order).  vr_max and rest of a cell are
-// written only by the wave that owns it, after it has read them.
+// Cells are handed out in chunks of a.chunk (<= 64) consecutive cells
+// through a counter (the work queue): waves that become resident late, or that drew
+// cheap cells, simply take more chunks.  The metadata of a chunk is loaded
+// one cell per lane with one vector load per field and read out with
+// v_readlane, so the per-cell loop issues no metadata load whose wait would
+// also drain the velocity prefetch (vmcnt counts in order).  vr_max and rest
+// of a cell are written only by the wave that owns it, after it has read
+// them.
 struct MetaBatch {
     uint32_t s, e, local, klo, khi;
     double coeff, vrm, rest;
 };
 
-__device__ __forceinline__ MetaBatch load_batch(const CollideArgs &a, uint32_t j0) {
+__device__ __forceinline__ uint32_t grab_chunk(uint32_t *queue) {
+    uint32_t v = 0;
+    if (threadIdx.x == 0) v = atomicAdd(queue, 1u);
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ MetaBatch load_batch(const CollideArgs &a, uint32_t chunk) {
     MetaBatch b{0u, 0u, 0u, 0u, 0u, 0.0, 0.0, 0.0};
-    const uint64_t c = (uint64_t)blockIdx.x + (uint64_t)(j0 + threadIdx.x) * gridDim.x;
-    if (c < a.ncells) {
+    const uint64_t c = (uint64_t)chunk * a.chunk + threadIdx.x;
+    if (threadIdx.x < a.chunk && c < a.ncells) {
         b.s = a.start[c];
         b.e = a.start[c + 1];
         const Member &M = a.members[a.cell_member[c]];
@@ -556,8 +567,7 @@ __device__ __forceinline__ CellMeta meta_at(const MetaBatch &b, uint32_t l, uint
 
 static constexpr int PREFETCH_CHUNKS = 16;   // velocities of up to 1024 particles in registers
 
-// Persistent waves: wave w handles cells w, w + G, w + 2G, ... (G waves in
-// the grid).  Per cell: wait for its velocities (prefetched one cell
+// Persistent waves take chunks of cells from the work queue.  Per cell: wait for its velocities (prefetched one cell
 // earlier) and put them into LDS; issue the loads of the next cell's
 // velocities; NTC collisions on LDS; write-back, FP64 moments (as
 // fire-and-forget atomics) and the cell's NTC state.  The loop's one
@@ -566,7 +576,6 @@ static constexpr int PREFETCH_CHUNKS = 16;   // velocities of up to 1024 particl
 // 1 KiB mark table.
 __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     const uint32_t lane = threadIdx.x;
-    const uint32_t G = gridDim.x;
     const uint32_t K = a.lds_cap;
     float *lx = dyn_lds, *ly = dyn_lds + K, *lz = dyn_lds + 2 * K;
     uint32_t *mark = (uint32_t *)(dyn_lds + 3 * K);
@@ -574,10 +583,13 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     const uint32_t step = *a.step;
     unsigned long long acc_total = 0, cand_total = 0, overflow = 0;
 
-    // cell j of this wave's list is blockIdx.x + j * G
-    MetaBatch cur = load_batch(a, 0), nxt = load_batch(a, WAVE);
+    // cell j of the current chunk q0 is q0 * chunk + j; chunk q1 is next
+    uint32_t q0 = grab_chunk(a.queue);
+    MetaBatch cur = load_batch(a, q0);
+    uint32_t q1 = grab_chunk(a.queue);
+    MetaBatch nxt = load_batch(a, q1);
     uint32_t j = 0;
-    CellMeta m0 = meta_at(cur, 0, blockIdx.x, a.ncells);
+    CellMeta m0 = meta_at(cur, 0, q0 * a.chunk, a.ncells);
     float px[PREFETCH_CHUNKS], py[PREFETCH_CHUNKS], pz[PREFETCH_CHUNKS];
 #define DSMC_PREFETCH(M)                                                                    \
     do {                                                                                    \
@@ -591,13 +603,17 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     DSMC_PREFETCH(m0);
 
     while (m0.valid) {
-        // the next cell's metadata; a new batch 64 cells ahead when this one ends
-        const uint32_t j1 = j + 1;
-        if ((j1 & (WAVE - 1)) == 0) {
+        // the next cell's metadata; at the end of a chunk, the next chunk
+        // becomes current and one more is taken from the queue
+        uint32_t j1 = j + 1;
+        if (j1 == a.chunk) {
             cur = nxt;
-            nxt = load_batch(a, j1 + WAVE);
+            q0 = q1;
+            q1 = grab_chunk(a.queue);
+            nxt = load_batch(a, q1);
+            j1 = 0;
         }
-        const CellMeta m1 = meta_at(cur, j1 & (WAVE - 1), blockIdx.x + j1 * G, a.ncells);
+        const CellMeta m1 = meta_at(cur, j1, q0 * a.chunk + j1, a.ncells);
         const uint32_t c = m0.c, s = m0.s, n = m0.n;
         const bool active = m0.coeff > 0.0;
         const bool in_lds = n <= K;
@@ -723,10 +739,13 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
         p[i] = v;
 }
 
+// mv_count[0]: movers of an unsorted buffer; mv_count[1]: the collide
+// kernel's work queue.
 __global__ void k_begin_step(uint32_t *step, uint32_t *mv_count) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         *step += 1u;
-        *mv_count = 0u;
+        mv_count[0] = 0u;
+        mv_count[1] = 0u;
     }
 }
 
@@ -808,8 +827,13 @@ hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, hipStream_t s) 
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_collide, 64, lds);
     if (e != hipSuccess) return e;
     uint32_t grid = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
-    if (grid > ncells) grid = ncells;
-    hipLaunchKernelGGL(k_collide, dim3(grid), dim3(64), lds, s, a);
+    // chunks: about eight per wave, at most 8 cells each
+    CollideArgs b = a;
+    uint32_t chunk = ncells / (grid * 8u);
+    b.chunk = chunk < 1u ? 1u : (chunk > 8u ? 8u : chunk);
+    const uint32_t nchunks = (ncells + b.chunk - 1) / b.chunk;
+    if (grid > nchunks) grid = nchunks;
+    hipLaunchKernelGGL(k_collide, dim3(grid), dim3(64), lds, s, b);
     LAUNCH_CHECK();
     return hipSuccess;
 }
