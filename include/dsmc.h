@@ -164,6 +164,25 @@ dsmc_status dsmc_get_collide_totals(dsmc_handle *h, double *tot8);
 enum { DSMC_OPT_TIMING = 6 };
 dsmc_status dsmc_get_kernel_times(dsmc_handle *h, int *nkernels, char *names, int names_len, double *ms);
 
+/* --- tuning (no reference counterpart: the reference has no launch
+ * parameters).  Both knobs leave results bit-identical. */
+enum {
+    DSMC_OPT_COLLIDE_CHUNK = 7, /* cells per collide work-queue entry, 1..64 (0: automatic) */
+    DSMC_OPT_GRAPH_STEPS = 8    /* steps per captured graph, even, 2..64 (default 2) */
+};
+typedef struct {
+    int graph_steps;            /* chosen steps per captured graph */
+    int collide_chunk;          /* chosen cells per collide queue entry (0: automatic) */
+    double ms_per_step;         /* best measured configuration */
+    double ms_per_step_default; /* the default configuration, same measurement */
+    int trials;                 /* configurations timed */
+} dsmc_tuning;
+/* Times each candidate configuration on the live system for steps_per_trial
+ * steps (graph replay, HIP events) and keeps the fastest.  The timed steps
+ * are real steps of the simulation (counters advance); since the knobs do
+ * not change results, a tuned run's trajectory equals an untuned one's. */
+dsmc_status dsmc_autotune(dsmc_handle *h, int steps_per_trial, dsmc_tuning *out);
+
 /* --- geometry (the step's input; complexgeometry.cpp) ------------------ */
 /* Reference packed-sphere generator (complexgeometry.cpp:401-513) and
  * classification (928-1063) into library-owned arrays. */

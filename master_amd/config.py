@@ -127,3 +127,36 @@ PRECISION = dict(n_vox=64, radius_vox=6.0, porosity=0.5, geom_seed=2, n_particle
 # 64^3 cells, 100M particles, Kn 0.05.
 ROOFLINE = dict(n_vox=256, rmin=3.0, rmax=10.0, porosity=0.45, geom_seed=3,
                 n_particles=100_000_000, kn=0.05, cells=64)
+
+
+# Batched ensemble: 64 independent porous boxes advanced by one handle.
+# Edge from {32, 48, 64, 96, 128} voxels, porosity U(0.3, 0.8), sphere radius
+# U(2, 8) voxels, Kn log-uniform in [0.01, 1], body force for a drift of
+# Mach log-uniform in [0.01, 0.1], ~8 particles per pore voxel (clamped to
+# 1e5..8e6 per member), seeds 100..163.  Deterministic in (n_members, seed0).
+ENSEMBLE_EDGES = (32, 48, 64, 96, 128)
+ENSEMBLE_MIN_N, ENSEMBLE_MAX_N = 100_000, 8_000_000     # particles per member
+
+
+def ensemble_members(n_members=64, seed0=100, per_voxel=8.0, edges=ENSEMBLE_EDGES):
+    import numpy as np
+    rng = np.random.default_rng(seed0)
+    out = []
+    for i in range(n_members):
+        out.append(dict(
+            edge=int(rng.choice(edges)),
+            porosity=float(rng.uniform(0.3, 0.8)),
+            radius=float(rng.uniform(2.0, 8.0)),
+            kn=float(10.0 ** rng.uniform(-2.0, 0.0)),
+            mach=float(10.0 ** rng.uniform(-2.0, -1.0)),
+            seed=seed0 + i,
+            per_voxel=per_voxel,
+        ))
+    return out
+
+
+def member_settings(spec, porosity):
+    """Settings of one ensemble member for its generated world's porosity."""
+    n = spec["edge"]
+    n_particles = int(min(max(spec["per_voxel"] * n ** 3 * porosity, ENSEMBLE_MIN_N), ENSEMBLE_MAX_N))
+    return dsmc_settings(n, porosity, n_particles, spec["kn"], n // 4, seed=spec["seed"], mach=spec["mach"])

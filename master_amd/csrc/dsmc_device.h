@@ -40,6 +40,7 @@ struct Member {
     float s_gas;         // sqrt(T / m)
     int wall_model;
     uint32_t key_lo, key_hi;
+    uint32_t pfirst;     // first particle of the member in cell order (members keep their particles)
 };
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).

@@ -74,6 +74,8 @@ struct dsmc_handle {
     int sample_every = 0;
     uint32_t lds_cap = 0;
     bool lds_auto = true;
+    uint32_t collide_chunk = 0;     // 0: automatic
+    int graph_steps = 2;            // steps per captured graph (even)
     // host mirrors of System counters
     int64_t steps = 0;
     double t = 0;
@@ -537,7 +539,7 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
     const bool collide = (h->phases & 4) != 0;
     if (collide || sample) {
         Scope sc(h, "k_collide");
-        HIPCHK(h, launch_collide(collide_args(h, collide, sample), h->ncells, s));
+        HIPCHK(h, launch_collide(collide_args(h, collide, sample), h->ncells, h->collide_chunk, s));
     }
     if (h->totals) {
         int b = h->cur;
@@ -548,13 +550,14 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
 }
 
 dsmc_status capture_graph(dsmc_handle *h, int parity, bool sample) {
-    // Two steps, so the buffers return to the parity the graph starts from.
+    // An even number of steps, so the buffers return to the parity the
+    // graph starts from.
     int save_cur = h->cur, save_scur = h->scur;
     bool save_timing = h->timing;
     h->timing = false;
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    dsmc_status st = enqueue_step(h, sample);
-    if (st == DSMC_OK) st = enqueue_step(h, sample);
+    dsmc_status st = DSMC_OK;
+    for (int k = 0; k < h->graph_steps && st == DSMC_OK; k++) st = enqueue_step(h, sample);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     h->timing = save_timing;
@@ -570,13 +573,27 @@ dsmc_status capture_graph(dsmc_handle *h, int parity, bool sample) {
     return DSMC_OK;
 }
 
-dsmc_status seed_members(dsmc_handle *h, const std::vector<int64_t> &counts) {
+// Member particle ranges: a member's particles stay in its own cells, so in
+// cell order they occupy the same index range every step.
+dsmc_status set_member_ranges(dsmc_handle *h, const std::vector<int64_t> &counts) {
     int64_t total = 0;
+    std::vector<Member> ms;
     for (size_t m = 0; m < h->mem.size(); m++) {
         h->mem[m].first = total;
         h->mem[m].count = counts[m];
+        h->mem[m].dev.pfirst = (uint32_t)total;
+        ms.push_back(h->mem[m].dev);
         total += counts[m];
     }
+    HIPCHK(h, hipMemcpy(h->d_members, ms.data(), ms.size() * sizeof(Member), hipMemcpyHostToDevice));
+    return DSMC_OK;
+}
+
+dsmc_status seed_members(dsmc_handle *h, const std::vector<int64_t> &counts) {
+    dsmc_status st0 = set_member_ranges(h, counts);
+    if (st0 != DSMC_OK) return st0;
+    int64_t total = 0;
+    for (size_t m = 0; m < h->mem.size(); m++) total += counts[m];
     dsmc_status st = alloc_particles(h, total);
     if (st != DSMC_OK) return st;
     std::vector<uint32_t> pm(total), first(h->mem.size());
@@ -816,7 +833,7 @@ dsmc_status dsmc_step(dsmc_handle *h, int nsteps) {
     int done = 0;
     while (done < nsteps) {
         bool sample = h->sample_every > 0 && ((h->steps + 1) % h->sample_every) == 0;
-        bool graph_ok = h->use_graph && !h->timing && nsteps - done >= 2 &&
+        bool graph_ok = h->use_graph && !h->timing && nsteps - done >= h->graph_steps &&
                         (h->sample_every == 0 || h->sample_every == 1);
         if (graph_ok) {
             int par = h->cur;
@@ -831,12 +848,12 @@ dsmc_status dsmc_step(dsmc_handle *h, int nsteps) {
                 if (st != DSMC_OK) return st;
             }
             HIPCHK(h, hipGraphLaunch(h->graph[par], h->stream));
-            for (int k = 0; k < 2; k++) {
+            for (int k = 0; k < h->graph_steps; k++) {
                 h->steps++;
                 h->t += h->mem[0].s.dt;
                 if (sample) h->nsamples++;
             }
-            done += 2;
+            done += h->graph_steps;
             continue;
         }
         dsmc_status st = enqueue_step(h, sample);
@@ -1025,7 +1042,7 @@ dsmc_status dsmc_set_option(dsmc_handle *h, int option, int64_t value) {
             h->lds_cap = auto_lds_cap(h->max_occ);
         } else {
             h->lds_auto = false;
-            h->lds_cap = (uint32_t)((value + 63) / 64 * 64);
+            h->lds_cap = (uint32_t)((value + 31) / 32 * 32);
         }
         return DSMC_OK;
     case DSMC_OPT_USE_GRAPH:
@@ -1034,6 +1051,16 @@ dsmc_status dsmc_set_option(dsmc_handle *h, int option, int64_t value) {
         return DSMC_OK;
     case DSMC_OPT_TIMING:
         h->timing = value != 0;
+        return DSMC_OK;
+    case DSMC_OPT_COLLIDE_CHUNK:
+        if (value < 0 || value > 64) return DSMC_ERR_INVALID_ARGUMENT;
+        h->collide_chunk = (uint32_t)value;
+        drop_graphs(h);
+        return DSMC_OK;
+    case DSMC_OPT_GRAPH_STEPS:
+        if (value < 2 || value > 64 || (value & 1)) return DSMC_ERR_INVALID_ARGUMENT;
+        h->graph_steps = (int)value;
+        drop_graphs(h);
         return DSMC_OK;
     default:
         return DSMC_ERR_INVALID_ARGUMENT;
@@ -1054,6 +1081,69 @@ dsmc_status dsmc_get_last_wall_hits(dsmc_handle *h, int32_t *nhits, int32_t *las
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (nhits) HIPCHK(h, hipMemcpy(nhits, h->d_rec_nhit, h->N * 4, hipMemcpyDeviceToHost));
     if (last_voxel) HIPCHK(h, hipMemcpy(last_voxel, h->d_rec_last, h->N * 4, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_autotune(dsmc_handle *h, int steps_per_trial, dsmc_tuning *out) {
+    if (!h || steps_per_trial < 1) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->initialized) return fail(h, DSMC_ERR_STATE, "autotune before initialize");
+    const bool save_graph = h->use_graph, save_timing = h->timing;
+    const int save_steps = h->graph_steps;
+    const uint32_t save_chunk = h->collide_chunk;
+    h->use_graph = true;
+    h->timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    struct Cand { int gs; uint32_t chunk; };
+    std::vector<Cand> cands;
+    const int gsteps[] = {2, 4, 8};
+    const uint32_t chunks[] = {0, 1, 2, 4, 8, 16};
+    for (int gs : gsteps)
+        for (uint32_t ch : chunks) cands.push_back({gs, ch});
+    double best = 1e300, dflt = 0.0;
+    Cand pick = {2, 0};
+    dsmc_status st = DSMC_OK;
+    for (const Cand &cd : cands) {
+        h->graph_steps = cd.gs;
+        h->collide_chunk = cd.chunk;
+        drop_graphs(h);
+        // a whole number of graph launches, the first one (capture) untimed
+        const int reps = (steps_per_trial + cd.gs - 1) / cd.gs;
+        st = dsmc_step(h, cd.gs);
+        if (st != DSMC_OK) break;
+        if (hipEventRecord(e0, h->stream) != hipSuccess) { st = fail(h, DSMC_ERR_HIP, "hipEventRecord"); break; }
+        st = dsmc_step(h, reps * cd.gs);
+        if (st != DSMC_OK) break;
+        if (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            st = fail(h, DSMC_ERR_HIP, "hipEventRecord/Synchronize");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double per = (double)ms / (reps * cd.gs);
+        if (cd.gs == 2 && cd.chunk == 0) dflt = per;
+        if (per < best) { best = per; pick = cd; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->use_graph = save_graph;
+    h->timing = save_timing;
+    drop_graphs(h);
+    if (st != DSMC_OK) {
+        h->graph_steps = save_steps;
+        h->collide_chunk = save_chunk;
+        return st;
+    }
+    h->graph_steps = pick.gs;
+    h->collide_chunk = pick.chunk;
+    if (out) {
+        out->graph_steps = pick.gs;
+        out->collide_chunk = (int)pick.chunk;
+        out->ms_per_step = best;
+        out->ms_per_step_default = dflt;
+        out->trials = (int)cands.size();
+    }
     return DSMC_OK;
 }
 

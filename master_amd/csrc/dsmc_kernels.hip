@@ -94,7 +94,9 @@ __global__ __launch_bounds__(64) void k_move(MoveArgs a) {
             int ix, iy, iz;
             unpack_vox(vp, ix, iy, iz);
             uint32_t nhit = 0;
-            int32_t last = move_particle(M, V, frames, step, p, ix, iy, iz, ox, oy, oz, vx, vy, vz,
+            // draws are keyed by the index inside the member, so a member of an
+            // ensemble steps exactly as it would alone
+            int32_t last = move_particle(M, V, frames, step, p - M.pfirst, ix, iy, iz, ox, oy, oz, vx, vy, vz,
                                          cnt, nhit);
             a.ox[p] = ox; a.oy[p] = oy; a.oz[p] = oz;
             a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
@@ -815,7 +817,7 @@ hipError_t launch_gather(const GatherArgs &a, uint32_t ncells, hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, hipStream_t s) {
+hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, uint32_t chunk_opt, hipStream_t s) {
     if (ncells == 0) return hipSuccess;
     size_t lds = (size_t)a.lds_cap * 12 + MARKS * 4;  // one cell's velocities, mark table
     // persistent waves: as many as fit on the device at once
@@ -827,10 +829,11 @@ hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, hipStream_t s) 
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_collide, 64, lds);
     if (e != hipSuccess) return e;
     uint32_t grid = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
-    // chunks: about eight per wave, at most 8 cells each
+    // chunks: about eight per wave, at most 8 cells each, unless chosen
     CollideArgs b = a;
     uint32_t chunk = ncells / (grid * 8u);
     b.chunk = chunk < 1u ? 1u : (chunk > 8u ? 8u : chunk);
+    if (chunk_opt) b.chunk = chunk_opt > (uint32_t)WAVE ? (uint32_t)WAVE : chunk_opt;
     const uint32_t nchunks = (ncells + b.chunk - 1) / b.chunk;
     if (grid > nchunks) grid = nchunks;
     hipLaunchKernelGGL(k_collide, dim3(grid), dim3(64), lds, s, b);

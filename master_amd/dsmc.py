@@ -24,6 +24,7 @@ I64 = ctypes.c_int64
 
 WALL_THERMAL, WALL_SPECULAR = 0, 1
 OPT_PHASES, OPT_RECORD, OPT_TOTALS, OPT_LDS_CAPACITY, OPT_USE_GRAPH, OPT_TIMING = 1, 2, 3, 4, 5, 6
+OPT_COLLIDE_CHUNK, OPT_GRAPH_STEPS = 7, 8
 PHASE_MOVE, PHASE_SORT, PHASE_COLLIDE = 1, 2, 4
 
 _STATUS = {0: "OK", 1: "invalid argument", 2: "configuration", 3: "capacity", 4: "HIP runtime",
@@ -58,6 +59,11 @@ class Counters(ctypes.Structure):
                 ("overflow_cells", ctypes.c_uint64)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("graph_steps", ctypes.c_int), ("collide_chunk", ctypes.c_int), ("ms_per_step", D),
+                ("ms_per_step_default", D), ("trials", ctypes.c_int)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -74,7 +80,8 @@ def lib():
                      "dsmc_get_last_wall_hits", "dsmc_get_collide_totals", "dsmc_get_kernel_times",
                      "dsmc_num_particles", "dsmc_settings_from_ini", "dsmc_world_read",
                      "dsmc_geometry_packed_spheres", "dsmc_geometry_poly_spheres", "dsmc_world_write",
-                     "dsmc_save_state_file", "dsmc_load_state_file", "dsmc_initialize_fields"):
+                     "dsmc_save_state_file", "dsmc_load_state_file", "dsmc_initialize_fields",
+                     "dsmc_autotune"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.dsmc_destroy.restype = None
         _lib.dsmc_world_free.restype = None
@@ -235,6 +242,13 @@ class System:
     def load_state(self, rv6):
         rv6 = np.ascontiguousarray(rv6, dtype=np.float64)
         self._call("dsmc_load_state", _ptr(rv6), I64(len(rv6)))
+
+    def autotune(self, steps_per_trial=8):
+        """Times the launch configurations on this system and keeps the
+        fastest (results do not depend on them)."""
+        t = Tuning()
+        self._call("dsmc_autotune", ctypes.c_int(steps_per_trial), ctypes.byref(t))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def step(self, n=1):
         self._call("dsmc_step", ctypes.c_int(n))
