@@ -1,6 +1,6 @@
 """Throughput of the DSMC step (move + sort + collide) on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload roofline|precision|cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload roofline|precision|cpu|ensemble]
 
 Workload (default "roofline"): 256^3 voxel periodic box of polydisperse
 spheres (radius log-uniform 3..10 voxels, porosity 0.45, seed 3), 64^3
@@ -66,6 +66,30 @@ def workload(name):
         w = dsmc.WorldData.packed_spheres(p["n_vox"], p["radius_vox"], p["porosity"], p["geom_seed"])
     ini = C.dsmc_settings(p["n_vox"], w.porosity, p["n_particles"], p["kn"], p["cells"], seed=1)
     return p, w, ini
+
+
+def ensemble_system(args):
+    """64 independent members (edges 32..128, porosity 0.3..0.8, radius 2..8,
+    Kn 0.01..1, Mach 0.01..0.1, seeds 100..163) in one handle, autotuned."""
+    from master_amd import config as C
+    from master_amd import dsmc
+    t_setup = time.time()
+    specs = C.ensemble_members()
+    inis, worlds = [], []
+    for sp in specs:
+        e, r = sp["edge"], sp["radius"]
+        w = dsmc.WorldData.poly_spheres((e, e, e), r, r, sp["porosity"], sp["seed"])
+        inis.append(C.member_settings(sp, w.porosity))
+        worlds.append(w)
+    g = dsmc.System(inis, worlds)
+    n = g.initialize()
+    g.set_sampling(1)
+    log(f"[bench] ensemble of {len(specs)} members, {n} particles, {g.ncells} cells "
+        f"({time.time() - t_setup:.1f}s)")
+    tune = g.autotune(8)
+    log(f"[bench] autotune: {tune}")
+    p = {"kn": "0.01-1 (64 members)"}
+    return p, worlds, g, n, tune
 
 
 def make_system(args):
@@ -217,14 +241,37 @@ def measure_traffic(args):
             "written": {k: v.get("WRITE_SIZE") for k, v in vals.items()}}, scale
 
 
+def bench_ensemble(args):
+    from master_amd import dsmc
+    p, worlds, g, n, tune = ensemble_system(args)
+    g.set_option(dsmc.OPT_USE_GRAPH, 1)
+    g.step(args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    c = g.counters()
+    value = n * args.steps / el
+    out = {"metric": "particle-steps/s (move+sort+collide), 64-member ensemble", "value": round(value, 1),
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(1000 * el / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"ensemble: 64 members, edges 32-128, {n} particles, {g.ncells} cells",
+                      "particles": n, "cells": g.ncells, "members": len(worlds), "parallelism": "single GPU"},
+           "autotune": tune, "errors": c["errors"]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="roofline", choices=["roofline", "precision", "cpu"])
+    ap.add_argument("--workload", default="roofline", choices=["roofline", "precision", "cpu", "ensemble"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--lds-cap", type=int, default=0, help="particles per cell held in LDS (0: automatic)")
@@ -241,6 +288,8 @@ def main():
         return
 
     from master_amd import dsmc
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
     p, w, g, n = make_system(args)
 
     # per-kernel device times (HIP events on the handle's stream), eager launches
@@ -251,6 +300,10 @@ def main():
     log(f"[bench] kernel ms/launch: {ktimes}")
 
     g.set_option(dsmc.OPT_USE_GRAPH, 1)
+    tune = None
+    if not args.no_autotune:
+        tune = g.autotune(8)
+        log(f"[bench] autotune: {tune}")
     g.step(args.warmup)
     g.synchronize()
     log("[bench] warmup done")
@@ -318,6 +371,7 @@ def main():
                    "particles": n, "voxels": list(w.n), "cells": g.ncells, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "autotune": tune,
     }
     print(json.dumps(out), flush=True)
 
