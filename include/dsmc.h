@@ -105,10 +105,14 @@ const char *dsmc_last_error(const dsmc_handle *h);
 dsmc_status dsmc_initialize(dsmc_handle *h, int64_t *n_out);
 /* Member-wise variant for ensembles: n_per_member[i] particles in member i. */
 dsmc_status dsmc_initialize_counts(dsmc_handle *h, const int64_t *n_per_member);
-/* Seed from per-cell fields (density scale, drift, temperature): 5 doubles
- * per cell (n_rel, ux, uy, uz, T_red) for member `member`; the number of
- * particles per cell is n_rel * N_mean_per_empty_voxel * empty voxels. */
-dsmc_status dsmc_initialize_fields(dsmc_handle *h, int member, const double *fields5, int64_t *n_out);
+/* Warm start (no reference counterpart): seed every member from per-cell
+ * fields, 5 doubles per cell in global cell order (n_rel, ux, uy, uz, T_red).
+ * A member keeps the particle count dsmc_initialize would give it; cell c
+ * receives a share proportional to n_rel[c] times its empty voxels (whole
+ * particles by cumulative rounding), positions uniform in its empty voxels,
+ * velocities Maxwellian with drift u and temperature T (T <= 0: the gas
+ * temperature).  Cells without empty voxels receive none. */
+dsmc_status dsmc_initialize_fields(dsmc_handle *h, const double *fields5, int64_t *n_out);
 
 /* --- state I/O: DSMC_IO::load/save_state_to_file_binary (dsmc_io.cpp:82-159) */
 /* rv6: per particle r0 r1 r2 v0 v1 v2 (doubles), the reference state layout. */

@@ -709,9 +709,98 @@ dsmc_status dsmc_initialize_counts(dsmc_handle *h, const int64_t *n_per_member) 
     return seed_members(h, counts);
 }
 
-dsmc_status dsmc_initialize_fields(dsmc_handle *h, int member, const double *fields5, int64_t *n_out) {
-    (void)member; (void)fields5; (void)n_out;
-    return fail(h, DSMC_ERR_STATE, "field seeding not available in this build");
+dsmc_status dsmc_initialize_fields(dsmc_handle *h, const double *fields5, int64_t *n_out) {
+    if (!h || !fields5) return DSMC_ERR_INVALID_ARGUMENT;
+    const uint32_t nc = h->ncells;
+    std::vector<uint32_t> first(nc + 1, 0);
+    std::vector<float> ux(nc), uy(nc), uz(nc), sd(nc);
+    std::vector<int64_t> counts;
+    std::vector<uint32_t> pm;
+    int64_t total = 0;
+    for (size_t mi = 0; mi < h->mem.size(); mi++) {
+        const MemberHost &m = h->mem[mi];
+        const int64_t N = reference_count(m);
+        if (m.s.max_molecules > 0 && N > m.s.max_molecules)
+            return fail(h, DSMC_ERR_CAPACITY, "more molecules than max_molecules_per_node (system.cpp:399-403)");
+        // empty voxels per cell and the cells' weights
+        std::vector<double> wgt(m.dev.ncell, 0.0);
+        double wsum = 0.0;
+        for (uint32_t lc = 0; lc < m.dev.ncell; lc++) {
+            const int cz = (int)(lc % (uint32_t)m.dev.cpa[2]);
+            const int cy = (int)((lc / (uint32_t)m.dev.cpa[2]) % (uint32_t)m.dev.cpa[1]);
+            const int cx = (int)(lc / ((uint32_t)m.dev.cpa[1] * (uint32_t)m.dev.cpa[2]));
+            int empty = 0;
+            for (int a = 0; a < m.dev.b[0]; a++)
+                for (int b = 0; b < m.dev.b[1]; b++)
+                    for (int c = 0; c < m.dev.b[2]; c++) {
+                        const size_t x = (size_t)cx * m.dev.b[0] + a, y = (size_t)cy * m.dev.b[1] + b,
+                                     z = (size_t)cz * m.dev.b[2] + c;
+                        empty += m.types[(x * m.n[1] + y) * m.n[2] + z] == 0 ? 1 : 0;
+                    }
+            const double *f = fields5 + 5 * (size_t)(m.dev.cell0 + lc);
+            for (int k = 0; k < 5; k++)
+                if (!std::isfinite(f[k])) return fail(h, DSMC_ERR_INVALID_ARGUMENT, "non-finite field value");
+            const double nrel = f[0] > 0.0 ? f[0] : 0.0;
+            wgt[lc] = empty > 0 ? nrel * empty : 0.0;
+            wsum += wgt[lc];
+            const uint32_t gc = m.dev.cell0 + lc;
+            ux[gc] = (float)f[1];
+            uy[gc] = (float)f[2];
+            uz[gc] = (float)f[3];
+            const double T = f[4] > 0.0 ? f[4] : m.s.temperature;
+            sd[gc] = (float)std::sqrt(T / m.s.mass);
+        }
+        if (!(wsum > 0.0)) return fail(h, DSMC_ERR_INVALID_ARGUMENT, "fields give no particles to a member");
+        // whole particles by cumulative rounding: the member total is exactly N
+        double cum = 0.0;
+        int64_t placed = 0;
+        for (uint32_t lc = 0; lc < m.dev.ncell; lc++) {
+            cum += wgt[lc];
+            const int64_t upto = (lc + 1 == m.dev.ncell) ? N : (int64_t)std::llround(cum / wsum * (double)N);
+            const int64_t k = std::max<int64_t>(0, upto - placed);
+            first[m.dev.cell0 + lc + 1] = (uint32_t)k;   // count for now
+            placed += k;
+        }
+        counts.push_back(placed);
+        pm.insert(pm.end(), (size_t)placed, (uint32_t)mi);
+        total += placed;
+    }
+    if (total > 0xffffffffll) return fail(h, DSMC_ERR_CAPACITY, "too many particles");
+    for (uint32_t c = 0; c < nc; c++) first[c + 1] += first[c];
+    dsmc_status st = set_member_ranges(h, counts);
+    if (st != DSMC_OK) return st;
+    st = alloc_particles(h, total);
+    if (st != DSMC_OK) return st;
+    uint32_t *d_first = nullptr;
+    float *d_f = nullptr;
+    HIPCHK(h, dalloc(&d_first, nc + 1));
+    HIPCHK(h, dalloc(&d_f, 4 * (size_t)nc));
+    HIPCHK(h, hipMemcpy(d_first, first.data(), (nc + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(d_f, ux.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(d_f + nc, uy.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(d_f + 2 * (size_t)nc, uz.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(d_f + 3 * (size_t)nc, sd.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    SeedFieldsArgs a;
+    int c = h->cur;
+    a.ox = h->ox[c]; a.oy = h->oy[c]; a.oz = h->oz[c]; a.vx = h->vx[c]; a.vy = h->vy[c]; a.vz = h->vz[c];
+    a.vox = h->vox[c];
+    a.first = d_first;
+    a.ux = d_f; a.uy = d_f + nc; a.uz = d_f + 2 * (size_t)nc; a.sd = d_f + 3 * (size_t)nc;
+    a.members = h->d_members;
+    a.cell_member = h->d_cell_member;
+    a.types = h->d_types;
+    a.counters = h->d_cnt;
+    a.n = (uint32_t)total;
+    a.ncells = nc;
+    HIPCHK(h, launch_seed_fields(a, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    dfree(d_f);
+    dfree(d_first);
+    st = sort_unsorted(h, pm);
+    if (st != DSMC_OK) return st;
+    h->initialized = true;
+    if (n_out) *n_out = total;
+    return DSMC_OK;
 }
 
 dsmc_status dsmc_load_state(dsmc_handle *h, const double *rv6, int64_t n) {

@@ -101,6 +101,22 @@ struct SeedArgs {
     uint32_t n;
 };
 
+// Seeding from per-cell fields: particles [first[c], first[c+1]) belong to
+// global cell c and get a position uniform in the cell's empty voxels and a
+// Maxwellian velocity of drift u[c] and spread sd[c] = sqrt(T[c] / m).
+struct SeedFieldsArgs {
+    float *ox, *oy, *oz, *vx, *vy, *vz;
+    uint32_t *vox;
+    const uint32_t *first;              // ncells + 1
+    const float *ux, *uy, *uz, *sd;     // per cell: drift, sqrt(T / m)
+    const Member *members;
+    const uint32_t *cell_member;
+    const uint8_t *types;
+    unsigned long long *counters;
+    uint32_t n, ncells;
+};
+
+hipError_t launch_seed_fields(const SeedFieldsArgs &a, hipStream_t s);
 hipError_t launch_move(const MoveArgs &a, uint32_t ncells, hipStream_t s);
 hipError_t launch_classify_all(const ClassifyArgs &a, hipStream_t s);
 hipError_t launch_scan(const ScanArgs &a, hipStream_t s);

@@ -735,6 +735,56 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
     }
 }
 
+// Warm start: particle p of cell c (binary search in first[]), drawn with
+// the member's key and the particle's index inside the member.
+__global__ __launch_bounds__(256) void k_seed_fields(SeedFieldsArgs a) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = a.ncells;             // first[lo] <= p < first[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.first[mid] <= p) lo = mid; else hi = mid;
+        }
+        const uint32_t c = lo;
+        const Member M = a.members[a.cell_member[c]];
+        const uint32_t i = p - M.pfirst;
+        const uint8_t *types = a.types + M.vox0;
+        uint4 o = philox(TAG_INIT_VEL, 1u, i, 0u, M.key_lo, M.key_hi);
+        float r01 = sqrtf(-2.0f * logf(u01f(o.x)));
+        float s01, c01, s2, c2;
+        sincospif(2.0f * u01f(o.y), &s01, &c01);
+        float r2 = sqrtf(-2.0f * logf(u01f(o.z)));
+        sincospif(2.0f * u01f(o.w), &s2, &c2);
+        const float sd = a.sd[c];     // sqrt(T / m) of the cell
+        a.vx[p] = a.ux[c] + (r01 * c01) * sd;
+        a.vy[p] = a.uy[c] + (r01 * s01) * sd;
+        a.vz[p] = a.uz[c] + (r2 * c2) * sd;
+        // position: uniform over the cell's empty voxels (rejection)
+        const uint32_t lc = c - M.cell0;
+        const int cz = (int)(lc % (uint32_t)M.cpa[2]);
+        const int cy = (int)((lc / (uint32_t)M.cpa[2]) % (uint32_t)M.cpa[1]);
+        const int cx = (int)(lc / ((uint32_t)M.cpa[1] * (uint32_t)M.cpa[2]));
+        const uint32_t bv = (uint32_t)(M.b[0] * M.b[1] * M.b[2]);
+        int ix = cx * M.b[0], iy = cy * M.b[1], iz = cz * M.b[2];
+        float ox = 0.5f, oy = 0.5f, oz = 0.5f;
+        bool ok = false;
+        for (uint32_t attempt = 0; attempt < 100000u && !ok; attempt++) {
+            const uint4 q = philox(TAG_INIT_POS, 1u, i, attempt, M.key_lo, M.key_hi);
+            const uint32_t k = __umulhi(q.x, bv);
+            const int jz = (int)(k % (uint32_t)M.b[2]);
+            const int jy = (int)((k / (uint32_t)M.b[2]) % (uint32_t)M.b[1]);
+            const int jx = (int)(k / ((uint32_t)M.b[1] * (uint32_t)M.b[2]));
+            const int x = cx * M.b[0] + jx, y = cy * M.b[1] + jy, z = cz * M.b[2] + jz;
+            if (types[((size_t)x * M.n[1] + y) * M.n[2] + z] != V_EMPTY) continue;
+            ix = x; iy = y; iz = z;
+            ox = u01f(q.y); oy = u01f(q.z); oz = u01f(q.w);
+            ok = true;
+        }
+        if (!ok) atomicAdd(&a.counters[CNT_ERROR], 1ull);
+        a.ox[p] = ox; a.oy[p] = oy; a.oz[p] = oz;
+        a.vox[p] = pack_vox(ix, iy, iz);
+    }
+}
+
 // ------------------------------------------------------------------ small kernels
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -837,6 +887,13 @@ hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, uint32_t chunk_
     const uint32_t nchunks = (ncells + b.chunk - 1) / b.chunk;
     if (grid > nchunks) grid = nchunks;
     hipLaunchKernelGGL(k_collide, dim3(grid), dim3(64), lds, s, b);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_seed_fields(const SeedFieldsArgs &a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_seed_fields, dim3(2048), dim3(256), 0, s, a);
     LAUNCH_CHECK();
     return hipSuccess;
 }

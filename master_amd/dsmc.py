@@ -239,6 +239,15 @@ class System:
         self._call("dsmc_initialize_counts", _ptr(c))
         return int(c.sum())
 
+    def initialize_fields(self, fields):
+        """Warm start from per-cell fields (ncells, 5): n_rel, ux, uy, uz, T."""
+        f = np.ascontiguousarray(fields, dtype=np.float64)
+        if f.shape != (self.ncells, 5):
+            raise ValueError(f"fields must be ({self.ncells}, 5), got {f.shape}")
+        n = I64()
+        self._call("dsmc_initialize_fields", _ptr(f), ctypes.byref(n))
+        return n.value
+
     def load_state(self, rv6):
         rv6 = np.ascontiguousarray(rv6, dtype=np.float64)
         self._call("dsmc_load_state", _ptr(rv6), I64(len(rv6)))
