@@ -1,0 +1,68 @@
+/*
+ * dsmc_unet.h -- C ABI of the warm-start surrogate in master_amd/libdsmc.so.
+ *
+ * No counterpart in the reference solver: the reference always starts from a
+ * uniform Maxwellian (system.cpp:398-430).  This network maps a porous
+ * geometry (solid mask on an edge^3 voxel grid) plus two scalars (a Knudsen
+ * and a body-force feature) to per-cell fields on the (edge/4)^3 collision
+ * cells: 5 outputs per cell, in the normalisation the caller trains with
+ * (the Python side uses n_rel - 1, u / u_scale (3), T / T_gas - 1).  The
+ * fields seed dsmc_initialize_fields.
+ *
+ * Architecture (a 3-D U-Net, 4 levels of 16/32/64/128 channels, periodic
+ * 3x3x3 convolutions with ReLU, 2x average pooling, nearest upsampling, one
+ * skip connection at the output level, a 1x1 head): BF16 weights and
+ * activations, FP32 accumulation on MFMA, FP32 master weights and Adam.
+ *
+ * Parameter vector layout (dsmc_unet_get_params / set_params): for each of
+ * the 10 convolutions in order, W[cout][kp] (k = tap * cin + ci with tap =
+ * (dz+1)*9 + (dy+1)*3 + (dx+1), entries k >= 27*cin are zero; kp = 27*cin
+ * rounded up to 32) then b[cout]; then the head W[5][64], b[5].
+ * Convolutions (cin -> cout @ edge): 8->16, 16->16 @ e; 16->32, 32->32 @ e/2;
+ * 32->64, 64->64 @ e/4; 64->128, 128->128 @ e/8; 192->64, 64->64 @ e/4 (the
+ * 192 inputs are the upsampled 128 channels followed by the 64 skip channels
+ * of the second e/4 convolution).  Input channels: 0 solid mask (1 = solid),
+ * 1 and 2 the two scalars, 3-7 zero.
+ */
+#ifndef DSMC_UNET_H
+#define DSMC_UNET_H
+#include <stdint.h>
+#include "dsmc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dsmc_unet dsmc_unet;
+
+/* edge: input cube edge, a multiple of 16; max_batch: samples per call. */
+dsmc_status dsmc_unet_create(int edge, int max_batch, uint64_t seed, dsmc_unet **out);
+void dsmc_unet_destroy(dsmc_unet *net);
+const char *dsmc_unet_last_error(const dsmc_unet *net);
+int64_t dsmc_unet_num_params(const dsmc_unet *net);
+dsmc_status dsmc_unet_get_params(dsmc_unet *net, float *params);
+dsmc_status dsmc_unet_set_params(dsmc_unet *net, const float *params);
+
+/* masks: batch x edge^3 bytes (z-fastest, 1 = solid); scalars: batch x 2;
+ * out: batch x (edge/4)^3 x 5 floats (cell order z-fastest). */
+dsmc_status dsmc_unet_forward(dsmc_unet *net, int batch, const uint8_t *masks, const float *scalars, float *out);
+
+/* Weighted mean-square loss L = sum_b,c w[b][c] sum_o (out - target)^2 /
+ * (5 sum w), its value, the forward output and dL/dparams (same layout as the
+ * parameter vector).  No update. */
+dsmc_status dsmc_unet_loss_grad(dsmc_unet *net, int batch, const uint8_t *masks, const float *scalars,
+                                const float *targets, const float *weights, double *loss, float *out,
+                                float *grads);
+
+/* Training set resident on the device (copied). */
+dsmc_status dsmc_unet_set_data(dsmc_unet *net, int count, const uint8_t *masks, const float *scalars,
+                               const float *targets, const float *weights);
+/* Adam (beta 0.9, 0.999, eps 1e-8) over shuffled mini-batches; epoch_loss
+ * (optional, `epochs` floats) receives each epoch's mean batch loss. */
+dsmc_status dsmc_unet_train(dsmc_unet *net, int epochs, int batch, float lr, uint64_t shuffle_seed,
+                            float *epoch_loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -26,7 +26,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
 
-LIB_SOURCES = ["dsmc_kernels.hip", "dsmc_host.cpp", "dsmc_io.cpp"]
+LIB_SOURCES = ["dsmc_kernels.hip", "dsmc_host.cpp", "dsmc_io.cpp", "unet_kernels.hip", "unet_host.cpp"]
 
 
 def _run(cmd, cwd=ROOT):

@@ -1,0 +1,454 @@
+// unet_host.cpp -- the warm-start surrogate network behind include/dsmc_unet.h:
+// buffers, forward, backward, Adam, training loop.  All tensors live in HBM;
+// the host only enqueues kernels (unet_kernels.hip) on the network's stream.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+#include "../../include/dsmc_unet.h"
+#include "unet_kernels.h"
+
+using unet::bf16;
+
+namespace {
+
+struct Conv {
+    int cin, cout, S;     // S: cube edge the layer runs at
+    int kp, kpb;          // padded K of the forward and of the data-gradient GEMM
+    size_t w_off, b_off;  // into the parameter vector
+    bf16 *wf = nullptr, *wb = nullptr;
+};
+
+constexpr int NCONV = 10;
+constexpr int HEAD_C = 64, HEAD_O = 5;
+
+int pad32(int k) { return (k + 31) / 32 * 32; }
+
+}  // namespace
+
+struct dsmc_unet {
+    std::string err;
+    hipStream_t stream = nullptr;
+    int S0 = 0, maxB = 0;
+    Conv conv[NCONV];
+    size_t head_w = 0, head_b = 0, nparams = 0;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    int64_t adam_t = 0;
+    // activations (bf16) and their gradients (f32), sized for maxB
+    bf16 *x0 = nullptr;
+    bf16 *act[NCONV] = {};
+    bf16 *pool1 = nullptr, *pool3 = nullptr, *pool5 = nullptr, *cat = nullptr;
+    float *dact[NCONV] = {};
+    float *dpool = nullptr, *dcat = nullptr, *dskip = nullptr, *dtmp = nullptr;
+    float *out = nullptr, *dout = nullptr;
+    // workspace
+    bf16 *dz = nullptr, *xcol = nullptr, *zt = nullptr;
+    float *part = nullptr;
+    float *loss_part = nullptr;
+    int loss_blocks = 64;
+    // batch inputs
+    uint8_t *bmask = nullptr;
+    float *bscal = nullptr, *btgt = nullptr, *bw = nullptr;
+    // resident training set
+    int ndata = 0;
+    uint8_t *dmask = nullptr;
+    float *dscal = nullptr, *dtgt = nullptr, *dwt = nullptr;
+    std::vector<double> wsum;
+};
+
+namespace {
+
+#define UCHK(n, call)                                                                \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            (n)->err = std::string(#call) + ": " + hipGetErrorString(e_);            \
+            return DSMC_ERR_HIP;                                                     \
+        }                                                                            \
+    } while (0)
+
+template <class T>
+hipError_t ualloc(T **p, size_t n) {
+    *p = nullptr;
+    return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+}
+template <class T>
+void ufree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+dsmc_status ufail(dsmc_unet *n, dsmc_status st, const std::string &m) {
+    if (n) n->err = m;
+    return st;
+}
+
+size_t vox(int S) { return (size_t)S * S * S; }
+
+// the input tensor of convolution i
+const bf16 *conv_input(const dsmc_unet *n, int i) {
+    switch (i) {
+    case 0: return n->x0;
+    case 2: return n->pool1;
+    case 4: return n->pool3;
+    case 6: return n->pool5;
+    case 8: return n->cat;
+    default: return n->act[i - 1];
+    }
+}
+
+dsmc_status forward(dsmc_unet *n, int B) {
+    hipStream_t s = n->stream;
+    const int S0 = n->S0;
+    UCHK(n, unet::launch_build_input(n->bmask, n->bscal, n->x0, (int)vox(S0), (int)(B * vox(S0)), s));
+    for (int i = 0; i < NCONV; i++) {
+        const Conv &c = n->conv[i];
+        if (i == 2) UCHK(n, unet::launch_pool_fwd(n->act[1], n->pool1, S0, B, 16, s));
+        if (i == 4) UCHK(n, unet::launch_pool_fwd(n->act[3], n->pool3, S0 / 2, B, 32, s));
+        if (i == 6) UCHK(n, unet::launch_pool_fwd(n->act[5], n->pool5, S0 / 4, B, 64, s));
+        if (i == 8) UCHK(n, unet::launch_up_concat_fwd(n->act[7], n->act[5], n->cat, S0 / 4, B, 128, 64, s));
+        unet::ConvArgs a;
+        a.x = conv_input(n, i);
+        a.w = c.wf;
+        a.bias = n->params + c.b_off;
+        a.y = n->act[i];
+        a.S = c.S;
+        a.M = (int)(B * vox(c.S));
+        a.cin = c.cin;
+        a.cout = c.cout;
+        a.kp = c.kp;
+        a.mode = 0;
+        UCHK(n, unet::launch_conv3(a, s));
+    }
+    const int M4 = (int)(B * vox(S0 / 4));
+    UCHK(n, unet::launch_head_fwd(n->act[9], n->params + n->head_w, n->params + n->head_b, n->out, M4, HEAD_C,
+                                  HEAD_O, s));
+    return DSMC_OK;
+}
+
+// loss of the batch in n->btgt / n->bw, gradients into n->grads
+dsmc_status backward(dsmc_unet *n, int B, double inv_norm, float *loss_slot) {
+    hipStream_t s = n->stream;
+    const int S0 = n->S0;
+    const int M4 = (int)(B * vox(S0 / 4));
+    UCHK(n, unet::launch_loss(n->out, n->btgt, n->bw, n->dout, loss_slot, M4, HEAD_O, (float)inv_norm,
+                              n->loss_blocks, s));
+    UCHK(n, unet::launch_head_bwd(n->dout, n->act[9], n->params + n->head_w, n->dact[9], n->grads + n->head_w,
+                                  n->grads + n->head_b, M4, HEAD_C, HEAD_O, s));
+    for (int i = NCONV - 1; i >= 0; i--) {
+        const Conv &c = n->conv[i];
+        const int M = (int)(B * vox(c.S));
+        const size_t na = (size_t)M * c.cout;
+        UCHK(n, unet::launch_relu_mask_cast(n->dact[i], n->act[i], n->dz, na, s));
+        UCHK(n, unet::launch_weight_grad(conv_input(n, i), n->dz, n->xcol, n->zt, n->part, n->grads + c.w_off,
+                                         n->grads + c.b_off, c.S, M, c.cin, c.cout, c.kp, s));
+        if (i == 0) break;
+        // data gradient: the same convolution with flipped, transposed weights
+        float *target = nullptr;
+        switch (i) {
+        case 2: case 4: case 6: target = n->dpool; break;
+        case 8: target = n->dcat; break;
+        default: target = n->dact[i - 1]; break;
+        }
+        UCHK(n, unet::launch_flip(c.wf, c.wb, c.cin, c.cout, c.kp, c.kpb, s));
+        unet::ConvArgs a;
+        a.x = n->dz;
+        a.w = c.wb;
+        a.bias = nullptr;
+        a.y = target;
+        a.S = c.S;
+        a.M = M;
+        a.cin = c.cout;
+        a.cout = c.cin;
+        a.kp = c.kpb;
+        a.mode = 1;
+        UCHK(n, unet::launch_conv3(a, s));
+        if (i == 2) UCHK(n, unet::launch_pool_bwd(n->dpool, n->dact[1], S0, B, 16, s));
+        if (i == 4) UCHK(n, unet::launch_pool_bwd(n->dpool, n->dact[3], S0 / 2, B, 32, s));
+        if (i == 6) {
+            UCHK(n, unet::launch_pool_bwd(n->dpool, n->dact[5], S0 / 4, B, 64, s));
+            UCHK(n, unet::launch_add(n->dact[5], n->dskip, (size_t)B * vox(S0 / 4) * 64, s));
+        }
+        if (i == 8) UCHK(n, unet::launch_up_concat_bwd(n->dcat, n->dact[7], n->dskip, S0 / 4, B, 128, 64, s));
+    }
+    return DSMC_OK;
+}
+
+dsmc_status cast_all(dsmc_unet *n) {
+    for (int i = 0; i < NCONV; i++) {
+        const Conv &c = n->conv[i];
+        UCHK(n, unet::launch_cast_weights(n->params + c.w_off, c.wf, c.cout, c.cin, c.kp, n->stream));
+    }
+    return DSMC_OK;
+}
+
+dsmc_status upload_batch(dsmc_unet *n, int B, const uint8_t *masks, const float *scal, const float *tgt,
+                         const float *w, hipMemcpyKind kind) {
+    const size_t v0 = vox(n->S0), v4 = vox(n->S0 / 4);
+    UCHK(n, hipMemcpyAsync(n->bmask, masks, B * v0, kind, n->stream));
+    UCHK(n, hipMemcpyAsync(n->bscal, scal, B * 2 * sizeof(float), kind, n->stream));
+    if (tgt) UCHK(n, hipMemcpyAsync(n->btgt, tgt, B * v4 * HEAD_O * sizeof(float), kind, n->stream));
+    if (w) UCHK(n, hipMemcpyAsync(n->bw, w, B * v4 * sizeof(float), kind, n->stream));
+    return DSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dsmc_unet_last_error(const dsmc_unet *n) { return n ? n->err.c_str() : "null network"; }
+
+void dsmc_unet_destroy(dsmc_unet *n) {
+    if (!n) return;
+    if (n->stream) (void)hipStreamSynchronize(n->stream);
+    for (auto &c : n->conv) { ufree(c.wf); ufree(c.wb); }
+    ufree(n->params); ufree(n->grads); ufree(n->adam_m); ufree(n->adam_v);
+    ufree(n->x0);
+    for (int i = 0; i < NCONV; i++) { ufree(n->act[i]); ufree(n->dact[i]); }
+    ufree(n->pool1); ufree(n->pool3); ufree(n->pool5); ufree(n->cat);
+    ufree(n->dpool); ufree(n->dcat); ufree(n->dskip); ufree(n->dtmp);
+    ufree(n->out); ufree(n->dout); ufree(n->dz); ufree(n->xcol); ufree(n->zt); ufree(n->part);
+    ufree(n->loss_part);
+    ufree(n->bmask); ufree(n->bscal); ufree(n->btgt); ufree(n->bw);
+    ufree(n->dmask); ufree(n->dscal); ufree(n->dtgt); ufree(n->dwt);
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    delete n;
+}
+
+static dsmc_status create_impl(dsmc_unet *n, int edge, int max_batch, uint64_t seed) {
+    n->S0 = edge;
+    n->maxB = max_batch;
+    const int e = edge;
+    const int spec[NCONV][3] = {{8, 16, e},       {16, 16, e},       {16, 32, e / 2}, {32, 32, e / 2},
+                                {32, 64, e / 4},  {64, 64, e / 4},   {64, 128, e / 8}, {128, 128, e / 8},
+                                {192, 64, e / 4}, {64, 64, e / 4}};
+    size_t off = 0;
+    for (int i = 0; i < NCONV; i++) {
+        Conv &c = n->conv[i];
+        c.cin = spec[i][0];
+        c.cout = spec[i][1];
+        c.S = spec[i][2];
+        c.kp = pad32(27 * c.cin);
+        c.kpb = pad32(27 * c.cout);
+        c.w_off = off;
+        off += (size_t)c.cout * c.kp;
+        c.b_off = off;
+        off += (size_t)c.cout;
+    }
+    n->head_w = off;
+    off += HEAD_O * HEAD_C;
+    n->head_b = off;
+    off += HEAD_O;
+    n->nparams = off;
+    // He-normal initialisation from a seeded generator; biases zero
+    std::vector<float> p(off, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (int i = 0; i < NCONV; i++) {
+        const Conv &c = n->conv[i];
+        std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (27.0f * c.cin)));
+        for (int co = 0; co < c.cout; co++)
+            for (int k = 0; k < 27 * c.cin; k++) p[c.w_off + (size_t)co * c.kp + k] = nd(rng);
+    }
+    {
+        std::normal_distribution<float> nd(0.0f, std::sqrt(1.0f / HEAD_C));
+        for (int k = 0; k < HEAD_O * HEAD_C; k++) p[n->head_w + k] = 0.1f * nd(rng);
+    }
+    UCHK(n, hipStreamCreate(&n->stream));
+    UCHK(n, ualloc(&n->params, off));
+    UCHK(n, ualloc(&n->grads, off));
+    UCHK(n, ualloc(&n->adam_m, off));
+    UCHK(n, ualloc(&n->adam_v, off));
+    UCHK(n, hipMemcpy(n->params, p.data(), off * sizeof(float), hipMemcpyHostToDevice));
+    UCHK(n, hipMemset(n->grads, 0, off * sizeof(float)));
+    UCHK(n, hipMemset(n->adam_m, 0, off * sizeof(float)));
+    UCHK(n, hipMemset(n->adam_v, 0, off * sizeof(float)));
+    const int B = max_batch;
+    size_t xcol_max = 0, zt_max = 0, part_max = 0, dz_max = 0;
+    for (int i = 0; i < NCONV; i++) {
+        Conv &c = n->conv[i];
+        UCHK(n, ualloc(&c.wf, (size_t)c.cout * c.kp));
+        UCHK(n, ualloc(&c.wb, (size_t)c.cin * c.kpb));
+        const int M = (int)(B * vox(c.S));
+        const size_t Mpad = (size_t)(M + 31) / 32 * 32;
+        UCHK(n, ualloc(&n->act[i], (size_t)M * c.cout));
+        UCHK(n, ualloc(&n->dact[i], (size_t)M * c.cout));
+        xcol_max = std::max(xcol_max, (size_t)c.kp * Mpad);
+        zt_max = std::max(zt_max, (size_t)c.cout * Mpad);
+        dz_max = std::max(dz_max, (size_t)M * c.cout);
+        for (int b = 1; b <= B; b++)     // any batch size up to max_batch
+            part_max = std::max(part_max, (size_t)unet::dw_splits((int)(b * vox(c.S)), c.cout, c.kp) * c.cout * c.kp);
+    }
+    UCHK(n, ualloc(&n->x0, (size_t)B * vox(e) * 8));
+    UCHK(n, ualloc(&n->pool1, (size_t)B * vox(e / 2) * 16));
+    UCHK(n, ualloc(&n->pool3, (size_t)B * vox(e / 4) * 32));
+    UCHK(n, ualloc(&n->pool5, (size_t)B * vox(e / 8) * 64));
+    UCHK(n, ualloc(&n->cat, (size_t)B * vox(e / 4) * 192));
+    UCHK(n, ualloc(&n->dpool, (size_t)B * vox(e / 2) * 16 > (size_t)B * vox(e / 4) * 32
+                                  ? (size_t)B * vox(e / 2) * 16 : (size_t)B * vox(e / 4) * 32));
+    UCHK(n, ualloc(&n->dcat, (size_t)B * vox(e / 4) * 192));
+    UCHK(n, ualloc(&n->dskip, (size_t)B * vox(e / 4) * 64));
+    UCHK(n, ualloc(&n->out, (size_t)B * vox(e / 4) * HEAD_O));
+    UCHK(n, ualloc(&n->dout, (size_t)B * vox(e / 4) * HEAD_O));
+    UCHK(n, ualloc(&n->dz, dz_max));
+    UCHK(n, ualloc(&n->xcol, xcol_max));
+    UCHK(n, ualloc(&n->zt, zt_max));
+    UCHK(n, ualloc(&n->part, part_max));
+    UCHK(n, ualloc(&n->loss_part, (size_t)n->loss_blocks * 4096));
+    UCHK(n, ualloc(&n->bmask, (size_t)B * vox(e)));
+    UCHK(n, ualloc(&n->bscal, (size_t)B * 2));
+    UCHK(n, ualloc(&n->btgt, (size_t)B * vox(e / 4) * HEAD_O));
+    UCHK(n, ualloc(&n->bw, (size_t)B * vox(e / 4)));
+    dsmc_status st = cast_all(n);
+    if (st != DSMC_OK) return st;
+    UCHK(n, hipStreamSynchronize(n->stream));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_unet_create(int edge, int max_batch, uint64_t seed, dsmc_unet **out) {
+    if (!out || edge < 16 || edge % 16 != 0 || edge > 256 || max_batch < 1 || max_batch > 64)
+        return DSMC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    dsmc_unet *n = new dsmc_unet;
+    dsmc_status st = create_impl(n, edge, max_batch, seed);
+    if (st != DSMC_OK) {
+        dsmc_unet_destroy(n);
+        return st;
+    }
+    *out = n;
+    return DSMC_OK;
+}
+
+int64_t dsmc_unet_num_params(const dsmc_unet *n) { return n ? (int64_t)n->nparams : 0; }
+
+dsmc_status dsmc_unet_get_params(dsmc_unet *n, float *p) {
+    if (!n || !p) return DSMC_ERR_INVALID_ARGUMENT;
+    UCHK(n, hipStreamSynchronize(n->stream));
+    UCHK(n, hipMemcpy(p, n->params, n->nparams * sizeof(float), hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_unet_set_params(dsmc_unet *n, const float *p) {
+    if (!n || !p) return DSMC_ERR_INVALID_ARGUMENT;
+    UCHK(n, hipStreamSynchronize(n->stream));
+    UCHK(n, hipMemcpy(n->params, p, n->nparams * sizeof(float), hipMemcpyHostToDevice));
+    UCHK(n, hipMemset(n->adam_m, 0, n->nparams * sizeof(float)));
+    UCHK(n, hipMemset(n->adam_v, 0, n->nparams * sizeof(float)));
+    n->adam_t = 0;
+    dsmc_status st = cast_all(n);
+    if (st != DSMC_OK) return st;
+    UCHK(n, hipStreamSynchronize(n->stream));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_unet_forward(dsmc_unet *n, int B, const uint8_t *masks, const float *scal, float *out) {
+    if (!n || !masks || !scal || !out || B < 1 || B > n->maxB) return DSMC_ERR_INVALID_ARGUMENT;
+    dsmc_status st = upload_batch(n, B, masks, scal, nullptr, nullptr, hipMemcpyHostToDevice);
+    if (st != DSMC_OK) return st;
+    st = forward(n, B);
+    if (st != DSMC_OK) return st;
+    UCHK(n, hipMemcpyAsync(out, n->out, (size_t)B * vox(n->S0 / 4) * HEAD_O * sizeof(float),
+                           hipMemcpyDeviceToHost, n->stream));
+    UCHK(n, hipStreamSynchronize(n->stream));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_unet_loss_grad(dsmc_unet *n, int B, const uint8_t *masks, const float *scal, const float *tgt,
+                                const float *w, double *loss, float *out, float *grads) {
+    if (!n || !masks || !scal || !tgt || !w || B < 1 || B > n->maxB) return DSMC_ERR_INVALID_ARGUMENT;
+    const size_t v4 = vox(n->S0 / 4);
+    double ws = 0.0;
+    for (size_t i = 0; i < (size_t)B * v4; i++) ws += w[i];
+    if (!(ws > 0.0)) return ufail(n, DSMC_ERR_INVALID_ARGUMENT, "loss weights sum to zero");
+    dsmc_status st = upload_batch(n, B, masks, scal, tgt, w, hipMemcpyHostToDevice);
+    if (st == DSMC_OK) st = forward(n, B);
+    if (st == DSMC_OK) st = backward(n, B, 1.0 / (HEAD_O * ws), n->loss_part);
+    if (st != DSMC_OK) return st;
+    std::vector<float> lp(n->loss_blocks);
+    UCHK(n, hipMemcpyAsync(lp.data(), n->loss_part, n->loss_blocks * sizeof(float), hipMemcpyDeviceToHost,
+                           n->stream));
+    if (out)
+        UCHK(n, hipMemcpyAsync(out, n->out, (size_t)B * v4 * HEAD_O * sizeof(float), hipMemcpyDeviceToHost,
+                               n->stream));
+    if (grads) UCHK(n, hipMemcpyAsync(grads, n->grads, n->nparams * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    UCHK(n, hipStreamSynchronize(n->stream));
+    if (loss) {
+        double L = 0.0;
+        for (float x : lp) L += x;
+        *loss = L;
+    }
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_unet_set_data(dsmc_unet *n, int count, const uint8_t *masks, const float *scal, const float *tgt,
+                               const float *w) {
+    if (!n || count < 1 || !masks || !scal || !tgt || !w) return DSMC_ERR_INVALID_ARGUMENT;
+    const size_t v0 = vox(n->S0), v4 = vox(n->S0 / 4);
+    ufree(n->dmask); ufree(n->dscal); ufree(n->dtgt); ufree(n->dwt);
+    UCHK(n, ualloc(&n->dmask, (size_t)count * v0));
+    UCHK(n, ualloc(&n->dscal, (size_t)count * 2));
+    UCHK(n, ualloc(&n->dtgt, (size_t)count * v4 * HEAD_O));
+    UCHK(n, ualloc(&n->dwt, (size_t)count * v4));
+    UCHK(n, hipMemcpy(n->dmask, masks, (size_t)count * v0, hipMemcpyHostToDevice));
+    UCHK(n, hipMemcpy(n->dscal, scal, (size_t)count * 2 * sizeof(float), hipMemcpyHostToDevice));
+    UCHK(n, hipMemcpy(n->dtgt, tgt, (size_t)count * v4 * HEAD_O * sizeof(float), hipMemcpyHostToDevice));
+    UCHK(n, hipMemcpy(n->dwt, w, (size_t)count * v4 * sizeof(float), hipMemcpyHostToDevice));
+    n->wsum.assign(count, 0.0);
+    for (int i = 0; i < count; i++)
+        for (size_t c = 0; c < v4; c++) n->wsum[i] += w[(size_t)i * v4 + c];
+    n->ndata = count;
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_unet_train(dsmc_unet *n, int epochs, int B, float lr, uint64_t shuffle_seed, float *epoch_loss) {
+    if (!n || epochs < 0 || B < 1 || B > n->maxB || !(lr > 0.0f)) return DSMC_ERR_INVALID_ARGUMENT;
+    if (n->ndata < B) return ufail(n, DSMC_ERR_STATE, "training set smaller than one batch");
+    const size_t v0 = vox(n->S0), v4 = vox(n->S0 / 4);
+    const int nb = n->ndata / B;
+    if (nb > 4096) return ufail(n, DSMC_ERR_INVALID_ARGUMENT, "more than 4096 batches per epoch");
+    std::vector<int> order(n->ndata);
+    std::vector<float> lp((size_t)nb * n->loss_blocks);
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < epochs; ep++) {
+        for (int i = 0; i < n->ndata; i++) order[i] = i;
+        std::mt19937_64 rng(shuffle_seed + 0x9E3779B97F4A7C15ull * (uint64_t)(ep + 1));
+        std::shuffle(order.begin(), order.end(), rng);
+        for (int bi = 0; bi < nb; bi++) {
+            double ws = 0.0;
+            for (int k = 0; k < B; k++) {
+                const int j = order[(size_t)bi * B + k];
+                ws += n->wsum[j];
+                UCHK(n, hipMemcpyAsync(n->bmask + k * v0, n->dmask + (size_t)j * v0, v0, hipMemcpyDeviceToDevice,
+                                       n->stream));
+                UCHK(n, hipMemcpyAsync(n->bscal + 2 * k, n->dscal + 2 * (size_t)j, 2 * sizeof(float),
+                                       hipMemcpyDeviceToDevice, n->stream));
+                UCHK(n, hipMemcpyAsync(n->btgt + k * v4 * HEAD_O, n->dtgt + (size_t)j * v4 * HEAD_O,
+                                       v4 * HEAD_O * sizeof(float), hipMemcpyDeviceToDevice, n->stream));
+                UCHK(n, hipMemcpyAsync(n->bw + k * v4, n->dwt + (size_t)j * v4, v4 * sizeof(float),
+                                       hipMemcpyDeviceToDevice, n->stream));
+            }
+            if (!(ws > 0.0)) continue;
+            dsmc_status st = forward(n, B);
+            if (st == DSMC_OK) st = backward(n, B, 1.0 / (HEAD_O * ws), n->loss_part + (size_t)bi * n->loss_blocks);
+            if (st != DSMC_OK) return st;
+            n->adam_t++;
+            const float c1 = 1.0f - std::pow(b1, (float)n->adam_t), c2 = 1.0f - std::pow(b2, (float)n->adam_t);
+            UCHK(n, unet::launch_adam(n->params, n->grads, n->adam_m, n->adam_v, n->nparams, lr, b1, b2, eps, c1,
+                                      c2, n->stream));
+            st = cast_all(n);
+            if (st != DSMC_OK) return st;
+        }
+        UCHK(n, hipMemcpyAsync(lp.data(), n->loss_part, lp.size() * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+        UCHK(n, hipStreamSynchronize(n->stream));
+        if (epoch_loss) {
+            double L = 0.0;
+            for (float x : lp) L += x;
+            epoch_loss[ep] = (float)(L / nb);
+        }
+    }
+    return DSMC_OK;
+}
+
+}  // extern "C"

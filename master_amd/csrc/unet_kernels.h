@@ -1,0 +1,59 @@
+// unet_kernels.h -- launchers of the surrogate network's kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace unet {
+
+typedef __bf16 bf16;
+
+// split of the weight-gradient reduction over voxels: about 4096 waves in
+// total, slices of whole 32-voxel steps, at most MAX_SPLIT slices
+static constexpr int MAX_SPLIT = 256;
+inline int dw_splits(int M, int cout, int kp) {
+    const int Mpad = (M + 31) / 32 * 32;
+    const int tiles = (cout / 16) * (kp / 16);
+    int nsplit = (4096 + tiles - 1) / tiles;
+    const int maxsplit = Mpad / 32;
+    if (nsplit > maxsplit) nsplit = maxsplit;
+    if (nsplit > MAX_SPLIT) nsplit = MAX_SPLIT;
+    if (nsplit < 1) nsplit = 1;
+    const int mslice = (Mpad / nsplit + 31) / 32 * 32;
+    return (Mpad + mslice - 1) / mslice;
+}
+
+struct ConvArgs {
+    const bf16 *x;        // [M][cin]
+    const bf16 *w;        // [cout][kp]
+    const float *bias;    // [cout] (mode 0)
+    void *y;              // [M][cout]: bf16 (mode 0) or float (mode 1)
+    int S, M;             // cube edge, batch * S^3
+    int cin, cout, kp;
+    int mode;             // 0: relu(acc + bias) as bf16; 1: acc as float
+};
+
+hipError_t launch_build_input(const uint8_t *mask, const float *scal, bf16 *x, int S3, int M, hipStream_t s);
+hipError_t launch_conv3(const ConvArgs &a, hipStream_t s);
+hipError_t launch_flip(const bf16 *wf, bf16 *wb, int cin, int cout, int kp, int kpb, hipStream_t s);
+hipError_t launch_weight_grad(const bf16 *x, const bf16 *dz, bf16 *xcol, bf16 *zt, float *part, float *dW,
+                              float *db, int S, int M, int cin, int cout, int kp, hipStream_t s);
+hipError_t launch_relu_mask_cast(const float *da, const bf16 *a, bf16 *dz, size_t n, hipStream_t s);
+hipError_t launch_add(float *a, const float *b, size_t n, hipStream_t s);
+hipError_t launch_pool_fwd(const bf16 *x, bf16 *y, int S, int B, int C, hipStream_t s);
+hipError_t launch_pool_bwd(const float *dy, float *dx, int S, int B, int C, hipStream_t s);
+hipError_t launch_up_concat_fwd(const bf16 *lo, const bf16 *skip, bf16 *y, int S, int B, int C1, int C2,
+                                hipStream_t s);
+hipError_t launch_up_concat_bwd(const float *dy, float *dlo, float *dskip, int S, int B, int C1, int C2,
+                                hipStream_t s);
+hipError_t launch_head_fwd(const bf16 *x, const float *W, const float *b, float *out, int M, int C, int O,
+                           hipStream_t s);
+hipError_t launch_head_bwd(const float *dout, const bf16 *x, const float *W, float *dx, float *dW, float *db,
+                           int M, int C, int O, hipStream_t s);
+hipError_t launch_loss(const float *out, const float *tgt, const float *w, float *dout, float *part, int M, int O,
+                       float inv_norm, int nblocks, hipStream_t s);
+hipError_t launch_adam(float *p, const float *g, float *m1, float *m2, size_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, hipStream_t s);
+hipError_t launch_cast_weights(const float *p, bf16 *wf, int cout, int cin, int kp, hipStream_t s);
+
+}  // namespace unet

@@ -1,0 +1,494 @@
+// unet_kernels.hip -- kernels of the warm-start surrogate (a 3-D U-Net) for
+// gfx950: BF16 activations and weights, FP32 accumulation on MFMA
+// (v_mfma_f32_16x16x32_bf16), periodic 3x3x3 convolutions.
+//
+// Layouts: activations channels-last [batch][z][y][x][C] (C contiguous);
+// convolution weights [cout][k] with k = tap * cin + ci, tap = (dz+1)*9 +
+// (dy+1)*3 + (dx+1), padded with zeros to kp (a multiple of 32).
+//
+// MFMA 16x16x32 bf16 operand layout (lane l, r = l % 16, g = l / 16):
+//   A (16 x 32): row r, k = 8g .. 8g+7;   B (32 x 16): column r, k = 8g .. 8g+7
+//   C (16 x 16): column r, rows 4g .. 4g+3.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "unet_kernels.h"
+
+namespace unet {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+__device__ __forceinline__ int wrap(int i, int S) { return i < 0 ? i + S : (i >= S ? i - S : i); }
+
+__device__ __forceinline__ float bf2f(bf16 v) { return (float)v; }
+__device__ __forceinline__ bf16 f2bf(float v) { return (bf16)v; }   // round to nearest even
+
+// ------------------------------------------------------------- input
+// x[m][0] = solid mask, x[m][1..2] = per-sample scalars, x[m][3..7] = 0
+__global__ void k_build_input(const uint8_t *mask, const float *scal, bf16 *x, int S3, int M) {
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const int b = m / S3;
+        bf16x8 v;
+        v[0] = f2bf(mask[m] ? 1.0f : 0.0f);
+        v[1] = f2bf(scal[2 * b]);
+        v[2] = f2bf(scal[2 * b + 1]);
+        for (int i = 3; i < 8; i++) v[i] = f2bf(0.0f);
+        *(bf16x8 *)(x + (size_t)m * 8) = v;
+    }
+}
+
+// ------------------------------------------------------------- conv (implicit GEMM)
+// One wave: 16 consecutive voxels x all output channels (NT tiles of 16).
+// Block: 4 waves.  mode 0: y = bf16(relu(acc + bias)); mode 1: y = f32 acc.
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m0 = (blockIdx.x * 4 + wv) * 16;
+    if (m0 >= a.M) return;
+    const int r = lane & 15, g = lane >> 4;
+    const int S = a.S, S3 = S * S * S;
+    const int m = m0 + r;
+    const bool row_ok = m < a.M;
+    int b = 0, z = 0, y = 0, xx = 0;
+    if (row_ok) {
+        b = m / S3;
+        const int rem = m - b * S3;
+        z = rem / (S * S);
+        y = (rem / S) % S;
+        xx = rem % S;
+    }
+    const int K = 27 * a.cin;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < a.kp; k0 += 32) {
+        const int k = k0 + 8 * g;
+        bf16x8 av;
+        for (int i = 0; i < 8; i++) av[i] = f2bf(0.0f);
+        if (row_ok && k < K) {
+            const int t = k / a.cin, ci = k - t * a.cin;
+            const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+            const size_t mm = (size_t)b * S3 + ((size_t)wrap(z + dz, S) * S + wrap(y + dy, S)) * S + wrap(xx + dx, S);
+            av = *(const bf16x8 *)(a.x + mm * a.cin + ci);
+        }
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const bf16x8 bv = *(const bf16x8 *)(a.w + (size_t)(j * 16 + r) * a.kp + k);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        const int col = j * 16 + r;
+        for (int i = 0; i < 4; i++) {
+            const int mo = m0 + 4 * g + i;
+            if (mo >= a.M) continue;
+            float v = acc[j][i];
+            if (a.mode == 0) {
+                v += a.bias[col];
+                ((bf16 *)a.y)[(size_t)mo * a.cout + col] = f2bf(v > 0.0f ? v : 0.0f);
+            } else {
+                ((float *)a.y)[(size_t)mo * a.cout + col] = v;
+            }
+        }
+    }
+}
+
+// weights for the data gradient: wb[ci][t'][co] = wf[co][26 - t'][ci]
+__global__ void k_flip(const bf16 *wf, bf16 *wb, int cin, int cout, int kp, int kpb) {
+    const int n = cin * kpb;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int ci = i / kpb, kk = i - ci * kpb;
+        bf16 v = f2bf(0.0f);
+        if (kk < 27 * cout) {
+            const int tp = kk / cout, co = kk - tp * cout;
+            v = wf[(size_t)co * kp + (26 - tp) * cin + ci];
+        }
+        wb[i] = v;
+    }
+}
+
+// ------------------------------------------------------------- weight gradient
+// xcol[k][m] = x[neighbour(m, tap)][ci], zero for k >= 27 cin or m >= M
+__global__ void k_im2col_t(const bf16 *x, bf16 *xcol, int S, int M, int Mpad, int cin, int kp) {
+    const int S3 = S * S * S;
+    const size_t n = (size_t)kp * Mpad;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / Mpad), m = (int)(i - (size_t)k * Mpad);
+        bf16 v = f2bf(0.0f);
+        if (m < M && k < 27 * cin) {
+            const int t = k / cin, ci = k - t * cin;
+            const int b = m / S3, rem = m - b * S3;
+            const int z = rem / (S * S), y = (rem / S) % S, xx = rem % S;
+            const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+            const size_t mm = (size_t)b * S3 + ((size_t)wrap(z + dz, S) * S + wrap(y + dy, S)) * S + wrap(xx + dx, S);
+            v = x[mm * cin + ci];
+        }
+        xcol[i] = v;
+    }
+}
+
+// zt[c][m] = z[m][c], zero for m >= M
+__global__ void k_transpose(const bf16 *z, bf16 *zt, int M, int Mpad, int C) {
+    const size_t n = (size_t)C * Mpad;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i / Mpad), m = (int)(i - (size_t)c * Mpad);
+        zt[i] = m < M ? z[(size_t)m * C + c] : f2bf(0.0f);
+    }
+}
+
+// part[s][co][k] = sum over the s-th slice of m of zt[co][m] * xcol[k][m].
+// One wave per (co tile, k tile, slice); A = zt rows, B = xcol rows, both
+// contiguous in m.
+__global__ __launch_bounds__(64) void k_gemm_dw(const bf16 *zt, const bf16 *xcol, float *part, int Mpad,
+                                                int cout, int kp, int mslice) {
+    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+    const int tiles_k = kp / 16;
+    const int tile = blockIdx.x;
+    const int co0 = (tile / tiles_k) * 16, k0 = (tile % tiles_k) * 16;
+    const int s = blockIdx.y;
+    const int mb = s * mslice, me = min(Mpad, mb + mslice);
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const bf16 *ar = zt + (size_t)(co0 + r) * Mpad;
+    const bf16 *br = xcol + (size_t)(k0 + r) * Mpad;
+    for (int m = mb; m < me; m += 32) {
+        const bf16x8 av = *(const bf16x8 *)(ar + m + 8 * g);
+        const bf16x8 bv = *(const bf16x8 *)(br + m + 8 * g);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    }
+    float *out = part + (size_t)s * cout * kp;
+    for (int i = 0; i < 4; i++) out[(size_t)(co0 + 4 * g + i) * kp + k0 + r] = acc[i];
+}
+
+// out[i] = sum_s part[s][i], summed in slice order (deterministic)
+__global__ void k_reduce_parts(const float *part, int nsplit, size_t n, float *out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int k = 0; k < nsplit; k++) s += part[(size_t)k * n + i];
+        out[i] = s;
+    }
+}
+
+// db[c] = sum_m zt[c][m]: one block per channel
+__global__ __launch_bounds__(256) void k_rowsum(const bf16 *zt, int Mpad, float *db) {
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float s = 0.0f;
+    for (int m = threadIdx.x; m < Mpad; m += 256) s += bf2f(zt[(size_t)c * Mpad + m]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = red[0];
+}
+
+// ------------------------------------------------------------- elementwise
+__global__ void k_relu_mask_cast(const float *da, const bf16 *a, bf16 *dz, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dz[i] = f2bf(bf2f(a[i]) > 0.0f ? da[i] : 0.0f);
+}
+
+__global__ void k_add(float *a, const float *b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        a[i] += b[i];
+}
+
+// 2x2x2 average; S is the fine edge
+__global__ void k_pool_fwd(const bf16 *x, bf16 *y, int S, int B, int C) {
+    const int s = S / 2;
+    const size_t n = (size_t)B * s * s * s * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const size_t v = i / C;
+        const int xx = (int)(v % s), y0 = (int)((v / s) % s), z = (int)((v / s / s) % s), b = (int)(v / s / s / s);
+        float acc = 0.0f;
+        for (int k = 0; k < 8; k++) {
+            const int fz = 2 * z + (k >> 2), fy = 2 * y0 + ((k >> 1) & 1), fx = 2 * xx + (k & 1);
+            acc += bf2f(x[((((size_t)b * S + fz) * S + fy) * S + fx) * C + c]);
+        }
+        y[i] = f2bf(acc * 0.125f);
+    }
+}
+
+__global__ void k_pool_bwd(const float *dy, float *dx, int S, int B, int C) {
+    const int s = S / 2;
+    const size_t n = (size_t)B * S * S * S * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const size_t v = i / C;
+        const int xx = (int)(v % S), y0 = (int)((v / S) % S), z = (int)((v / S / S) % S), b = (int)(v / S / S / S);
+        dx[i] = 0.125f * dy[((((size_t)b * s + z / 2) * s + y0 / 2) * s + xx / 2) * C + c];
+    }
+}
+
+// y[fine voxel] = [lo[coarse parent] (C1), skip[fine voxel] (C2)]; S fine edge
+__global__ void k_up_concat_fwd(const bf16 *lo, const bf16 *skip, bf16 *y, int S, int B, int C1, int C2) {
+    const int s = S / 2, C = C1 + C2;
+    const size_t n = (size_t)B * S * S * S * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const size_t v = i / C;
+        const int xx = (int)(v % S), y0 = (int)((v / S) % S), z = (int)((v / S / S) % S), b = (int)(v / S / S / S);
+        y[i] = c < C1 ? lo[((((size_t)b * s + z / 2) * s + y0 / 2) * s + xx / 2) * C1 + c] : skip[v * C2 + (c - C1)];
+    }
+}
+
+// dlo[coarse] = sum of the 8 children's first C1 channels; dskip = the rest
+__global__ void k_up_concat_bwd(const float *dy, float *dlo, float *dskip, int S, int B, int C1, int C2) {
+    const int s = S / 2, C = C1 + C2;
+    const size_t n1 = (size_t)B * s * s * s * C1;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n1; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C1);
+        const size_t v = i / C1;
+        const int xx = (int)(v % s), y0 = (int)((v / s) % s), z = (int)((v / s / s) % s), b = (int)(v / s / s / s);
+        float acc = 0.0f;
+        for (int k = 0; k < 8; k++) {
+            const int fz = 2 * z + (k >> 2), fy = 2 * y0 + ((k >> 1) & 1), fx = 2 * xx + (k & 1);
+            acc += dy[((((size_t)b * S + fz) * S + fy) * S + fx) * C + c];
+        }
+        dlo[i] = acc;
+    }
+    const size_t n2 = (size_t)B * S * S * S * C2;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t v = i / C2;
+        const int c = (int)(i % C2);
+        dskip[i] = dy[v * C + C1 + c];
+    }
+}
+
+// ------------------------------------------------------------- head (1x1, FP32)
+__global__ void k_head_fwd(const bf16 *x, const float *W, const float *bias, float *out, int M, int C, int O) {
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        for (int o = 0; o < O; o++) {
+            float s = bias[o];
+            for (int c = 0; c < C; c++) s += W[o * C + c] * bf2f(x[(size_t)m * C + c]);
+            out[(size_t)m * O + o] = s;
+        }
+    }
+}
+
+__global__ void k_head_bwd_x(const float *dout, const float *W, float *dx, int M, int C, int O) {
+    const size_t n = (size_t)M * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const size_t m = i / C;
+        float s = 0.0f;
+        for (int o = 0; o < O; o++) s += dout[m * O + o] * W[o * C + c];
+        dx[i] = s;
+    }
+}
+
+// dW[o][c] and db[o]: one block per (o, c) pair plus O blocks for the bias
+__global__ __launch_bounds__(256) void k_head_bwd_w(const float *dout, const bf16 *x, float *dW, float *db, int M,
+                                                    int C, int O) {
+    __shared__ float red[256];
+    const int j = blockIdx.x;
+    const bool is_b = j >= O * C;
+    const int o = is_b ? j - O * C : j / C, c = is_b ? 0 : j % C;
+    float s = 0.0f;
+    for (int m = threadIdx.x; m < M; m += 256)
+        s += dout[(size_t)m * O + o] * (is_b ? 1.0f : bf2f(x[(size_t)m * C + c]));
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (is_b) db[o] = red[0]; else dW[o * C + c] = red[0];
+    }
+}
+
+// weighted MSE: loss = sum_m w[m] sum_o (out - tgt)^2 * inv_norm;
+// dout = 2 w (out - tgt) inv_norm.  part[block] = the block's share of loss.
+__global__ __launch_bounds__(256) void k_loss(const float *out, const float *tgt, const float *w, float *dout,
+                                              float *part, int M, int O, float inv_norm) {
+    __shared__ float red[256];
+    float s = 0.0f;
+    for (int m = blockIdx.x * 256 + threadIdx.x; m < M; m += gridDim.x * 256) {
+        for (int o = 0; o < O; o++) {
+            const size_t i = (size_t)m * O + o;
+            const float d = out[i] - tgt[i];
+            s += w[m] * d * d;
+            dout[i] = 2.0f * w[m] * d * inv_norm;
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0] * inv_norm;
+}
+
+// ------------------------------------------------------------- optimiser
+__global__ void k_adam(float *p, const float *g, float *m1, float *m2, size_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float a = b1 * m1[i] + (1.0f - b1) * gi;
+        const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+        m1[i] = a;
+        m2[i] = b;
+        p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+    }
+}
+
+// FP32 master weights [cout][kp] (k = tap * cin + ci, zero beyond 27 cin)
+// -> BF16 copy for the GEMMs
+__global__ void k_cast_weights(const float *p, bf16 *wf, int cout, int cin, int kp) {
+    const int n = cout * kp;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int k = i % kp;
+        wf[i] = k < 27 * cin ? f2bf(p[i]) : f2bf(0.0f);
+    }
+}
+
+// ------------------------------------------------------------- launchers
+static inline int blocks_for(size_t n, int per = 256, int cap = 8192) {
+    size_t b = (n + per - 1) / per;
+    if (b < 1) b = 1;
+    return (int)(b > (size_t)cap ? cap : b);
+}
+
+#define UNET_LAUNCH_CHECK()                          \
+    do {                                             \
+        hipError_t e_ = hipGetLastError();           \
+        if (e_ != hipSuccess) return e_;             \
+    } while (0)
+
+hipError_t launch_build_input(const uint8_t *mask, const float *scal, bf16 *x, int S3, int M, hipStream_t s) {
+    hipLaunchKernelGGL(k_build_input, dim3(blocks_for(M)), dim3(256), 0, s, mask, scal, x, S3, M);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_conv3(const ConvArgs &a, hipStream_t s) {
+    if (a.cin % 8 != 0 || a.cout % 16 != 0 || a.kp % 32 != 0 || a.kp < 27 * a.cin) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((a.M + 63) / 64)), block(256);
+    switch (a.cout / 16) {
+    case 1: hipLaunchKernelGGL(k_conv3<1>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_conv3<2>, grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_conv3<4>, grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_conv3<8>, grid, block, 0, s, a); break;
+    case 12: hipLaunchKernelGGL(k_conv3<12>, grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_flip(const bf16 *wf, bf16 *wb, int cin, int cout, int kp, int kpb, hipStream_t s) {
+    hipLaunchKernelGGL(k_flip, dim3(blocks_for((size_t)cin * kpb)), dim3(256), 0, s, wf, wb, cin, cout, kp, kpb);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_weight_grad(const bf16 *x, const bf16 *dz, bf16 *xcol, bf16 *zt, float *part, float *dW,
+                              float *db, int S, int M, int cin, int cout, int kp, hipStream_t s) {
+    const int Mpad = (M + 31) / 32 * 32;
+    hipLaunchKernelGGL(k_im2col_t, dim3(blocks_for((size_t)kp * Mpad)), dim3(256), 0, s, x, xcol, S, M, Mpad, cin, kp);
+    UNET_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_transpose, dim3(blocks_for((size_t)cout * Mpad)), dim3(256), 0, s, dz, zt, M, Mpad, cout);
+    UNET_LAUNCH_CHECK();
+    // slices of m (dw_splits): enough waves to fill the device
+    const int tiles = (cout / 16) * (kp / 16);
+    int nsplit = (4096 + tiles - 1) / tiles;
+    const int maxsplit = Mpad / 32;
+    if (nsplit > maxsplit) nsplit = maxsplit;
+    if (nsplit > MAX_SPLIT) nsplit = MAX_SPLIT;
+    if (nsplit < 1) nsplit = 1;
+    const int mslice = (Mpad / nsplit + 31) / 32 * 32;
+    nsplit = dw_splits(M, cout, kp);
+    hipLaunchKernelGGL(k_gemm_dw, dim3(tiles, nsplit), dim3(64), 0, s, zt, xcol, part, Mpad, cout, kp, mslice);
+    UNET_LAUNCH_CHECK();
+    const size_t n = (size_t)cout * kp;
+    hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(n)), dim3(256), 0, s, part, nsplit, n, dW);
+    UNET_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_rowsum, dim3(cout), dim3(256), 0, s, zt, Mpad, db);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_relu_mask_cast(const float *da, const bf16 *a, bf16 *dz, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_relu_mask_cast, dim3(blocks_for(n)), dim3(256), 0, s, da, a, dz, n);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_add(float *a, const float *b, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_add, dim3(blocks_for(n)), dim3(256), 0, s, a, b, n);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_pool_fwd(const bf16 *x, bf16 *y, int S, int B, int C, hipStream_t s) {
+    const size_t n = (size_t)B * (S / 2) * (S / 2) * (S / 2) * C;
+    hipLaunchKernelGGL(k_pool_fwd, dim3(blocks_for(n)), dim3(256), 0, s, x, y, S, B, C);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_pool_bwd(const float *dy, float *dx, int S, int B, int C, hipStream_t s) {
+    const size_t n = (size_t)B * S * S * S * C;
+    hipLaunchKernelGGL(k_pool_bwd, dim3(blocks_for(n)), dim3(256), 0, s, dy, dx, S, B, C);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_up_concat_fwd(const bf16 *lo, const bf16 *skip, bf16 *y, int S, int B, int C1, int C2,
+                                hipStream_t s) {
+    const size_t n = (size_t)B * S * S * S * (C1 + C2);
+    hipLaunchKernelGGL(k_up_concat_fwd, dim3(blocks_for(n)), dim3(256), 0, s, lo, skip, y, S, B, C1, C2);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_up_concat_bwd(const float *dy, float *dlo, float *dskip, int S, int B, int C1, int C2,
+                                hipStream_t s) {
+    const size_t n = (size_t)B * S * S * S * C2;
+    hipLaunchKernelGGL(k_up_concat_bwd, dim3(blocks_for(n)), dim3(256), 0, s, dy, dlo, dskip, S, B, C1, C2);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_head_fwd(const bf16 *x, const float *W, const float *b, float *out, int M, int C, int O,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(M)), dim3(256), 0, s, x, W, b, out, M, C, O);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_head_bwd(const float *dout, const bf16 *x, const float *W, float *dx, float *dW, float *db,
+                           int M, int C, int O, hipStream_t s) {
+    hipLaunchKernelGGL(k_head_bwd_x, dim3(blocks_for((size_t)M * C)), dim3(256), 0, s, dout, W, dx, M, C, O);
+    UNET_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_head_bwd_w, dim3(O * C + O), dim3(256), 0, s, dout, x, dW, db, M, C, O);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_loss(const float *out, const float *tgt, const float *w, float *dout, float *part, int M, int O,
+                       float inv_norm, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_loss, dim3(nblocks), dim3(256), 0, s, out, tgt, w, dout, part, M, O, inv_norm);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_adam(float *p, const float *g, float *m1, float *m2, size_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, hipStream_t s) {
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(n)), dim3(256), 0, s, p, g, m1, m2, n, lr, b1, b2, eps, c1, c2);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_cast_weights(const float *p, bf16 *wf, int cout, int cin, int kp, hipStream_t s) {
+    hipLaunchKernelGGL(k_cast_weights, dim3(blocks_for((size_t)cout * kp)), dim3(256), 0, s, p, wf, cout, cin, kp);
+    UNET_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+}  // namespace unet

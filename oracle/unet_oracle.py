@@ -1,0 +1,205 @@
+"""CPU restatement of the warm-start surrogate network (TEST INFRASTRUCTURE).
+
+Only tests/ use this module: it is the checker of master_amd/libdsmc.so's
+U-Net kernels, never the thing trained or shipped.  It follows the network
+and parameter layout documented in include/dsmc_unet.h and the rounding
+points of the GPU kernels: inputs, weights, post-ReLU activations, pooled
+activations and the masked gradients fed to the convolution GEMMs are
+rounded to bfloat16 (round to nearest even); everything else is float64
+here (FP32 on the GPU).  With bf16=False all rounding is off, which is
+what the finite-difference checks of this module's own backward use.
+
+Parity of this restatement is pinned by: finite differences of the loss
+against its analytic gradient (exact mode), numpy reference convolution
+against a brute-force loop on a tiny grid, and the forward/backward of the
+library on identical inputs and parameters (tests/test_surrogate.py).
+"""
+import numpy as np
+
+CONVS = [(8, 16, 1), (16, 16, 1), (16, 32, 2), (32, 32, 2), (32, 64, 4), (64, 64, 4),
+         (64, 128, 8), (128, 128, 8), (192, 64, 4), (64, 64, 4)]     # (cin, cout, edge divisor)
+HEAD_C, HEAD_O = 64, 5
+
+
+def pad32(k):
+    return (k + 31) // 32 * 32
+
+
+def layout():
+    """(w_off, b_off, kp) per convolution, head offsets, parameter count."""
+    off, out = 0, []
+    for cin, cout, _ in CONVS:
+        kp = pad32(27 * cin)
+        w = off
+        off += cout * kp
+        b = off
+        off += cout
+        out.append((w, b, kp))
+    hw = off
+    off += HEAD_O * HEAD_C
+    hb = off
+    off += HEAD_O
+    return out, hw, hb, off
+
+
+def bf16(x):
+    """Round to bfloat16 (nearest even), returned as float64."""
+    f = np.asarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+TAPS = [(t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1) for t in range(27)]
+
+
+def conv3(x, W):
+    """Periodic 3x3x3 convolution.  x: (B, S, S, S, cin); W: (cout, 27, cin)."""
+    B, S = x.shape[0], x.shape[1]
+    y = np.zeros((B, S, S, S, W.shape[0]))
+    for t, (d0, d1, d2) in enumerate(TAPS):
+        xs = np.roll(x, shift=(-d0, -d1, -d2), axis=(1, 2, 3))    # xs[a] = x[a + d]
+        y += xs @ W[:, t, :].T
+    return y
+
+
+def conv3_data_grad(dz, W):
+    """d/dx of sum(dz * conv3(x, W)): x-gradient, (B, S, S, S, cin)."""
+    dx = np.zeros(dz.shape[:4] + (W.shape[2],))
+    for t, (d0, d1, d2) in enumerate(TAPS):
+        g = dz @ W[:, t, :]                                       # at output voxel a, for input a + d
+        dx += np.roll(g, shift=(d0, d1, d2), axis=(1, 2, 3))
+    return dx
+
+
+def conv3_weight_grad(dz, x):
+    """dW[co, t, ci] = sum_a dz[a, co] x[a + d_t, ci]."""
+    cout, cin = dz.shape[-1], x.shape[-1]
+    dW = np.zeros((cout, 27, cin))
+    dzf = dz.reshape(-1, cout)
+    for t, (d0, d1, d2) in enumerate(TAPS):
+        xs = np.roll(x, shift=(-d0, -d1, -d2), axis=(1, 2, 3)).reshape(-1, cin)
+        dW[:, t, :] = dzf.T @ xs
+    return dW
+
+
+def pool(x):
+    B, S, C = x.shape[0], x.shape[1], x.shape[-1]
+    return x.reshape(B, S // 2, 2, S // 2, 2, S // 2, 2, C).mean(axis=(2, 4, 6))
+
+
+def pool_grad(dy):
+    return np.repeat(np.repeat(np.repeat(dy, 2, 1), 2, 2), 2, 3) / 8.0
+
+
+def up(x):
+    return np.repeat(np.repeat(np.repeat(x, 2, 1), 2, 2), 2, 3)
+
+
+def up_grad(dy):
+    B, S, C = dy.shape[0], dy.shape[1], dy.shape[-1]
+    return dy.reshape(B, S // 2, 2, S // 2, 2, S // 2, 2, C).sum(axis=(2, 4, 6))
+
+
+def unpack(params):
+    lay, hw, hb, n = layout()
+    p = np.asarray(params, dtype=np.float64)
+    assert p.size == n
+    convs = []
+    for (cin, cout, _), (w, b, kp) in zip(CONVS, lay):
+        W = p[w:w + cout * kp].reshape(cout, kp)[:, :27 * cin].reshape(cout, 27, cin)
+        convs.append((W, p[b:b + cout]))
+    return convs, p[hw:hw + HEAD_O * HEAD_C].reshape(HEAD_O, HEAD_C), p[hb:hb + HEAD_O]
+
+
+def pack(conv_grads, dWh, dbh):
+    lay, hw, hb, n = layout()
+    g = np.zeros(n)
+    for (cin, cout, _), (w, b, kp), (dW, db) in zip(CONVS, lay, conv_grads):
+        full = np.zeros((cout, kp))
+        full[:, :27 * cin] = dW.reshape(cout, 27 * cin)
+        g[w:w + cout * kp] = full.reshape(-1)
+        g[b:b + cout] = db
+    g[hw:hw + HEAD_O * HEAD_C] = dWh.reshape(-1)
+    g[hb:hb + HEAD_O] = dbh
+    return g
+
+
+def forward_backward(params, masks, scalars, targets=None, weights=None, edge=16, use_bf16=True):
+    """masks: (B, e^3) 0/1; scalars: (B, 2); targets: (B, (e/4)^3, 5);
+    weights: (B, (e/4)^3).  Returns out (B, (e/4)^3, 5) and, with targets,
+    (loss, gradient vector)."""
+    R = bf16 if use_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    convs, Wh, bh = unpack(params)
+    Wq = [(R(W), b) for W, b in convs]
+    B, e = masks.shape[0], edge
+    x0 = np.zeros((B, e, e, e, 8))
+    x0[..., 0] = masks.reshape(B, e, e, e)
+    x0[..., 1] = scalars[:, 0][:, None, None, None]
+    x0[..., 2] = scalars[:, 1][:, None, None, None]
+    x0 = R(x0)
+    acts, ins = [], []
+
+    def layer(i, x):
+        W, b = Wq[i]
+        ins.append(x)
+        a = R(np.maximum(conv3(x, W) + b, 0.0))
+        acts.append(a)
+        return a
+
+    a = layer(0, x0)
+    a = layer(1, a)
+    a = layer(2, R(pool(a)))
+    a = layer(3, a)
+    a = layer(4, R(pool(a)))
+    a5 = layer(5, a)
+    a = layer(6, R(pool(a5)))
+    a7 = layer(7, a)
+    cat = np.concatenate([up(a7), a5], axis=-1)
+    a = layer(8, cat)
+    a9 = layer(9, a)
+    s4 = e // 4
+    out = (a9.reshape(B, s4 ** 3, HEAD_C) @ Wh.T) + bh
+    if targets is None:
+        return out
+    w = np.asarray(weights, dtype=np.float64)
+    norm = 1.0 / (HEAD_O * w.sum())
+    d = out - targets
+    loss = float((w[..., None] * d * d).sum() * norm)
+    dout = 2.0 * w[..., None] * d * norm
+    dWh = dout.reshape(-1, HEAD_O).T @ a9.reshape(-1, HEAD_C)
+    dbh = dout.reshape(-1, HEAD_O).sum(0)
+    da = (dout @ Wh).reshape(a9.shape)
+    grads = [None] * len(CONVS)
+    dskip = None
+    for i in range(len(CONVS) - 1, -1, -1):
+        W, _ = Wq[i]
+        dz = R(da * (acts[i] > 0))
+        grads[i] = (conv3_weight_grad(dz, ins[i]), dz.reshape(-1, dz.shape[-1]).sum(0))
+        if i == 0:
+            break
+        dx = conv3_data_grad(dz, W)
+        if i in (2, 4, 6):
+            da = pool_grad(dx)
+            if i == 6:
+                da = da + dskip
+        elif i == 8:
+            da, dskip = up_grad(dx[..., :128]), dx[..., 128:]
+        else:
+            da = dx
+    return out, loss, pack(grads, dWh, dbh)
+
+
+def init_params(seed, scale=1.0):
+    """Random parameters in the library layout (for tests)."""
+    lay, hw, hb, n = layout()
+    rng = np.random.default_rng(seed)
+    p = np.zeros(n)
+    for (cin, cout, _), (w, b, kp) in zip(CONVS, lay):
+        W = np.zeros((cout, kp))
+        W[:, :27 * cin] = rng.normal(0, scale * np.sqrt(2.0 / (27 * cin)), (cout, 27 * cin))
+        p[w:w + cout * kp] = W.reshape(-1)
+        p[b:b + cout] = rng.normal(0, 0.05, cout)
+    p[hw:hw + HEAD_O * HEAD_C] = rng.normal(0, 0.1, HEAD_O * HEAD_C)
+    p[hb:hb + HEAD_O] = rng.normal(0, 0.1, HEAD_O)
+    return p

@@ -1,0 +1,120 @@
+"""The warm-start surrogate's hand-written kernels against the CPU oracle
+(oracle/unet_oracle.py): forward outputs and every parameter gradient on
+identical inputs and parameters; training reduces the loss."""
+import numpy as np
+import pytest
+
+from oracle import unet_oracle as U
+
+EDGE, BATCH = 16, 2
+
+
+def _case(seed=3):
+    rng = np.random.default_rng(seed)
+    cells = (EDGE // 4) ** 3
+    masks = (rng.random((BATCH, EDGE ** 3)) < 0.4).astype(np.uint8)
+    scal = rng.normal(size=(BATCH, 2)).astype(np.float32)
+    tgt = rng.normal(size=(BATCH, cells, 5)).astype(np.float32)
+    w = rng.random((BATCH, cells)).astype(np.float32)
+    return masks, scal, tgt, w
+
+
+def test_oracle_gradient_matches_finite_differences():
+    """Exact mode (no bf16 rounding): analytic gradient of the oracle's loss
+    equals central differences on the parameters with the largest
+    gradients of every layer."""
+    masks, scal, tgt, w = _case()
+    p = U.init_params(1)
+    _, L, g = U.forward_backward(p, masks, scal, tgt, w, edge=EDGE, use_bf16=False)
+    lay, hw, hb, n = U.layout()
+    bounds = [(wo, bo + U.CONVS[i][1]) for i, (wo, bo, kp) in enumerate(lay)]
+    bounds.append((hw, hb + U.HEAD_O))
+    checked = 0
+    for lo, hi in bounds:
+        i = lo + int(np.argmax(np.abs(g[lo:hi])))
+        if g[i] == 0.0:
+            continue
+        eps = 1e-6
+        pp = p.copy()
+        pp[i] += eps
+        _, Lp, _ = U.forward_backward(pp, masks, scal, tgt, w, edge=EDGE, use_bf16=False)
+        pp[i] -= 2 * eps
+        _, Lm, _ = U.forward_backward(pp, masks, scal, tgt, w, edge=EDGE, use_bf16=False)
+        fd = (Lp - Lm) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-5 * max(abs(g[i]), 1e-3), (i, g[i], fd)
+        checked += 1
+    assert checked >= 8
+
+
+def test_oracle_convolution_brute_force():
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(1, 4, 4, 4, 3))
+    W = rng.normal(size=(2, 27, 3))
+    y = U.conv3(x, W)
+    yb = np.zeros_like(y)
+    for a in range(4):
+        for b in range(4):
+            for c in range(4):
+                for t, (d0, d1, d2) in enumerate(U.TAPS):
+                    yb[0, a, b, c] += W[:, t, :] @ x[0, (a + d0) % 4, (b + d1) % 4, (c + d2) % 4]
+    assert np.allclose(y, yb, atol=1e-12)
+    dz = rng.normal(size=y.shape)
+    assert np.isclose((dz * y).sum(), (U.conv3_data_grad(dz, W) * x).sum())
+    assert np.isclose((dz * y).sum(), (U.conv3_weight_grad(dz, x) * W).sum())
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.gpu
+def test_network_forward_and_gradients_match_oracle(gpu):
+    from master_amd import surrogate as S
+    net = S.Net(edge=EDGE, max_batch=BATCH, seed=0)
+    assert net.nparams == U.layout()[3]
+    p = U.init_params(7).astype(np.float32)
+    net.set_params(p)
+    masks, scal, tgt, w = _case()
+    out_g = net.forward(masks, scal)
+    loss_g, out_g2, g_g = net.loss_grad(masks, scal, tgt, w)
+    out_o, loss_o, g_o = U.forward_backward(p.astype(np.float64), masks, scal, tgt, w, edge=EDGE)
+    assert np.array_equal(out_g, out_g2)
+    # BF16 rounding points are the same; FP32 vs FP64 accumulation moves a
+    # few roundings by one bf16 ulp (2^-8 relative), and such differences
+    # grow along the backward chain: a few percent at the first layers, while
+    # a wrong tap, transpose or missing path would be O(1)
+    assert _rel(out_g, out_o) < 1e-2
+    assert abs(loss_g - loss_o) < 1e-2 * abs(loss_o)
+    lay, hw, hb, n = U.layout()
+    errs = {}
+    for i, (wo, bo, kp) in enumerate(lay):
+        cout = U.CONVS[i][1]
+        for name, sl in (("W", slice(wo, bo)), ("b", slice(bo, bo + cout))):
+            a, b = g_g[sl].astype(np.float64), g_o[sl]
+            errs[f"{name}{i}"] = (_rel(a, b), float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b))))
+    print({k: (round(e, 4), round(c, 5)) for k, (e, c) in errs.items()})
+    for k, (e, c) in errs.items():
+        assert e < 6e-2 and c > 0.998, (k, e, c)
+    assert _rel(g_g[hw:], g_o[hw:]) < 1e-2
+    net.close()
+
+
+@pytest.mark.gpu
+def test_training_reduces_loss(gpu):
+    """A small fit: targets a smooth function of the mask and scalars."""
+    from master_amd import surrogate as S
+    rng = np.random.default_rng(5)
+    n = 16
+    cells = (EDGE // 4) ** 3
+    masks = (rng.random((n, EDGE ** 3)) < 0.4).astype(np.uint8)
+    scal = rng.normal(size=(n, 2)).astype(np.float32)
+    frac = masks.reshape(n, 4, 4, 4, 4, 4, 4).transpose(0, 1, 3, 5, 2, 4, 6).reshape(n, cells, 64).mean(-1)
+    tgt = np.stack([frac - 0.4, 0.5 * scal[:, :1] * (1 - frac), 0 * frac, 0.2 * scal[:, 1:2] + 0 * frac,
+                    0.1 * frac], -1).astype(np.float32)
+    w = np.ones((n, cells), np.float32)
+    net = S.Net(edge=EDGE, max_batch=4, seed=1)
+    net.set_data(masks, scal, tgt, w)
+    losses = net.train(epochs=40, batch=4, lr=2e-3, seed=0)
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.3 * losses[0], losses
+    net.close()
