@@ -172,3 +172,167 @@ def fields_from_outputs(out, ini, mach):
     f[:, 1:4] = out[:, 1:4] * us
     f[:, 4] = T * np.clip(1.0 + out[:, 4], 0.5, 1.5)
     return f
+
+
+# ------------------------------------------------------------------ data, training, evaluation
+# Geometry and gas distribution the surrogate is trained on: config-3-style
+# random spheres on a 64^3 periodic grid (16^3 cells), restricted to the
+# range where cold-start DSMC reaches its steady flow within a few hundred
+# steps (Kn 0.3..1 against the voxel edge, pores of a few voxels) and where
+# the drift stands out of the per-cell sampling noise (forcing Mach >= 0.03).
+DOMAIN = dict(porosity=(0.3, 0.6), radius=(2.0, 5.0), kn=(0.3, 1.0), mach=(0.03, 0.1))
+EDGE = 64
+
+
+def sample_specs(n, seed0, domain=DOMAIN, edge=EDGE):
+    rng = np.random.default_rng(seed0)
+    out = []
+    for i in range(n):
+        out.append(dict(edge=edge, porosity=float(rng.uniform(*domain["porosity"])),
+                        radius=float(rng.uniform(*domain["radius"])),
+                        kn=float(10 ** rng.uniform(math.log10(domain["kn"][0]), math.log10(domain["kn"][1]))),
+                        mach=float(10 ** rng.uniform(math.log10(domain["mach"][0]), math.log10(domain["mach"][1]))),
+                        seed=int(seed0 + i), per_voxel=8.0))
+    return out
+
+
+def build_member(spec, particle_seed=None):
+    """World, settings, network input (solid mask, scalars) and empty
+    voxels per cell of one geometry."""
+    e, r = spec["edge"], spec["radius"]
+    w = dsmc.WorldData.poly_spheres((e, e, e), r, r, spec["porosity"], spec["seed"])
+    sp = dict(spec)
+    if particle_seed is not None:
+        sp["seed"] = particle_seed
+    ini = C.member_settings(sp, w.porosity)
+    types = np.asarray(w.types)
+    mask = (types != 0).astype(np.uint8)
+    return dict(world=w, ini=ini, mask=mask, scalars=features(spec["kn"], spec["mach"]),
+                empty=cell_geometry(types, e // 4), spec=spec)
+
+
+def _ensemble(members):
+    g = dsmc.System([m["ini"] for m in members], [m["world"] for m in members])
+    return g
+
+
+def _member_moments(mom, nmem, cells):
+    return mom.reshape(nmem, cells, 7)
+
+
+def generate(specs, t_eq=600, t_avg=3000, chunk=64, log=print):
+    """Steady per-cell fields of cold-start DSMC for each geometry: run
+    t_eq steps, then average t_avg steps.  Returns masks, scalars, targets,
+    weights (the network's training arrays).  The drift targets' sampling
+    noise scales as 1 / Mach of the forcing, so a field's loss weight
+    carries (Mach / Mach_max)^2 (inverse noise variance)."""
+    cells = (specs[0]["edge"] // 4) ** 3
+    masks, scal, tgts, wts = [], [], [], []
+    for i0 in range(0, len(specs), chunk):
+        mem = [build_member(s) for s in specs[i0:i0 + chunk]]
+        g = _ensemble(mem)
+        counts = None
+        n = g.initialize()
+        g.set_sampling(1)
+        g.set_option(dsmc.OPT_USE_GRAPH, 1)
+        g.step(t_eq)
+        g.reset_moments()
+        g.step(t_avg)
+        mom, ns = g.moments()
+        mom = _member_moments(mom, len(mem), cells)
+        for k, m in enumerate(mem):
+            nk = int(round(mom[k, :, 0].sum() / ns))
+            t, w = targets_from_moments(mom[k], ns, m["empty"], nk, m["ini"], m["spec"]["mach"])
+            w = w * (m["spec"]["mach"] / DOMAIN["mach"][1]) ** 2
+            masks.append(m["mask"])
+            scal.append(m["scalars"])
+            tgts.append(t)
+            wts.append(w)
+        g.close()
+        log(f"[surrogate] generated {min(i0 + chunk, len(specs))}/{len(specs)} fields ({n} particles in the batch)")
+        del counts
+    return (np.array(masks), np.array(scal, np.float32), np.array(tgts, np.float32), np.array(wts, np.float32))
+
+
+def drift_windows(g, nmem, cells, steps, window):
+    """Mean x-drift per member over consecutive windows of `window` steps."""
+    out = []
+    for _ in range(steps // window):
+        g.reset_moments()
+        g.step(window)
+        mom, _ = g.moments()
+        mom = _member_moments(mom, nmem, cells)
+        out.append(mom[:, :, 1].sum(1) / mom[:, :, 0].sum(1))
+    return np.array(out)          # (windows, members)
+
+
+def evaluate(net, specs, replicas=4, steps=1000, window=100, t_ss=(1000, 4000), log=print):
+    """Cold start vs surrogate warm start on held-out geometries.
+
+    Each geometry runs `replicas` times with different particle seeds; the
+    steady drift of a geometry is the cold runs' mean over steps t_ss.  The
+    error after window k is |u(k) - u_ss| / |u_ss| over the vector of
+    geometries (replica means of the window drift, normalised per geometry by
+    its forcing scale).  Returns the error curves, the steady drifts and the
+    per-step wall times of both runs."""
+    import time
+    members = []
+    for r in range(replicas):
+        for s in specs:
+            members.append(build_member(s, particle_seed=s["seed"] * 16 + r + 1))
+    nm, cells = len(members), (specs[0]["edge"] // 4) ** 3
+    us = np.array([u_scale(m["ini"], m["spec"]["mach"]) for m in members])
+    # cold
+    g = _ensemble(members)
+    g.initialize()
+    g.set_sampling(1)
+    g.set_option(dsmc.OPT_USE_GRAPH, 1)
+    g.synchronize()
+    t0 = time.perf_counter()
+    cold = drift_windows(g, nm, cells, steps, window)
+    g.synchronize()
+    cold_ms = 1000 * (time.perf_counter() - t0) / steps
+    g.step(t_ss[0] - steps)
+    ss = drift_windows(g, nm, cells, t_ss[1] - t_ss[0], window).mean(0)
+    g.close()
+    log(f"[surrogate] cold runs done ({nm} members, {cold_ms:.3f} ms/step)")
+    # warm
+    t0 = time.perf_counter()
+    fields = []
+    for i0 in range(0, len(specs), net.max_batch):
+        ss_ = specs[i0:i0 + net.max_batch]
+        mm = [members[i0 + k] for k in range(len(ss_))]
+        out = net.forward(np.array([m["mask"] for m in mm]), np.array([m["scalars"] for m in mm]))
+        for k, m in enumerate(mm):
+            fields.append(fields_from_outputs(out[k], m["ini"], m["spec"]["mach"]))
+    infer_s = time.perf_counter() - t0
+    fields = fields * replicas
+    g = _ensemble(members)
+    g.initialize_fields(np.concatenate(fields))
+    g.set_sampling(1)
+    g.set_option(dsmc.OPT_USE_GRAPH, 1)
+    g.synchronize()
+    t0 = time.perf_counter()
+    warm = drift_windows(g, nm, cells, steps, window)
+    g.synchronize()
+    warm_ms = 1000 * (time.perf_counter() - t0) / steps
+    g.close()
+    log(f"[surrogate] warm runs done ({warm_ms:.3f} ms/step, inference {infer_s:.3f} s)")
+    G = len(specs)
+
+    def err(curves):
+        u = curves.reshape(len(curves), replicas, G).mean(1) / us[:G]
+        ref = ss.reshape(replicas, G).mean(0) / us[:G]
+        return np.linalg.norm(u - ref, axis=1) / np.linalg.norm(ref)
+
+    return dict(cold_err=err(cold), warm_err=err(warm), window=window, cold_ms=cold_ms, warm_ms=warm_ms,
+                infer_s=infer_s, steady=(ss.reshape(replicas, G).mean(0) / us[:G]))
+
+
+def steps_to(err, window, tol):
+    """End step of the first window after which the error stays <= tol."""
+    ok = err <= tol
+    for k in range(len(ok)):
+        if ok[k:].all():
+            return (k + 1) * window
+    return None

@@ -118,3 +118,23 @@ def test_training_reduces_loss(gpu):
     assert np.all(np.isfinite(losses))
     assert losses[-1] < 0.3 * losses[0], losses
     net.close()
+
+
+@pytest.mark.gpu
+def test_trained_surrogate_warm_start_beats_cold_start(gpu):
+    """What the trained network gains: on held-out geometries (seeds 5000+,
+    never trained on), DSMC seeded from the surrogate's fields starts much
+    closer to the steady flow than the uniform Maxwellian seeding of
+    System::setup_molecules, and stays closer over the first windows."""
+    import os
+    from master_amd import surrogate as S
+    p = np.load(os.path.join(os.path.dirname(S.__file__), "surrogate_params.npy")).astype(np.float32)
+    net = S.Net(edge=S.EDGE, max_batch=4, seed=0)
+    net.set_params(p)
+    specs = S.sample_specs(8, 5000)
+    ev = S.evaluate(net, specs, replicas=2, steps=300, window=100, t_ss=(1000, 2500), log=print)
+    cold, warm = ev["cold_err"], ev["warm_err"]
+    print("cold", np.round(cold, 3), "warm", np.round(warm, 3))
+    assert warm[0] < 0.5 * cold[0]
+    assert warm.mean() < 0.6 * cold.mean()
+    net.close()
