@@ -169,19 +169,30 @@ __global__ void k_reduce_parts(const float *part, int nsplit, size_t n, float *o
     }
 }
 
-// db[c] = sum_m zt[c][m]: one block per channel
-__global__ __launch_bounds__(256) void k_rowsum(const bf16 *zt, int Mpad, float *db) {
+// db[c] = sum_m zt[c][m] in two deterministic passes: block (c, j) sums
+// chunk j of row c into part[c][j], then one thread per channel adds them.
+static constexpr int ROWSUM_CHUNK = 8192;
+__global__ __launch_bounds__(256) void k_rowsum_part(const bf16 *zt, int Mpad, float *part, int nchunk) {
     __shared__ float red[256];
-    const int c = blockIdx.x;
+    const int c = blockIdx.x, j = blockIdx.y;
+    const int m0 = j * ROWSUM_CHUNK, m1 = min(Mpad, m0 + ROWSUM_CHUNK);
     float s = 0.0f;
-    for (int m = threadIdx.x; m < Mpad; m += 256) s += bf2f(zt[(size_t)c * Mpad + m]);
+    for (int m = m0 + threadIdx.x; m < m1; m += 256) s += bf2f(zt[(size_t)c * Mpad + m]);
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[c] = red[0];
+    if (threadIdx.x == 0) part[(size_t)c * nchunk + j] = red[0];
+}
+
+__global__ void k_rowsum_final(const float *part, int nchunk, int C, float *db) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    float s = 0.0f;
+    for (int j = 0; j < nchunk; j++) s += part[(size_t)c * nchunk + j];
+    db[c] = s;
 }
 
 // ------------------------------------------------------------- elementwise
@@ -408,7 +419,13 @@ hipError_t launch_weight_grad(const bf16 *x, const bf16 *dz, bf16 *xcol, bf16 *z
     const size_t n = (size_t)cout * kp;
     hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(n)), dim3(256), 0, s, part, nsplit, n, dW);
     UNET_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_rowsum, dim3(cout), dim3(256), 0, s, zt, Mpad, db);
+    // the weight-gradient partials are consumed: reuse their buffer (it
+    // holds at least cout * kp >= cout * nchunk floats)
+    const int nchunk = (Mpad + ROWSUM_CHUNK - 1) / ROWSUM_CHUNK;
+    if ((size_t)nchunk > (size_t)kp) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rowsum_part, dim3(cout, nchunk), dim3(256), 0, s, zt, Mpad, part, nchunk);
+    UNET_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_rowsum_final, dim3((cout + 63) / 64), dim3(64), 0, s, part, nchunk, cout, db);
     UNET_LAUNCH_CHECK();
     return hipSuccess;
 }

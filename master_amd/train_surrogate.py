@@ -42,10 +42,26 @@ def main():
     ap.add_argument("--replicas", type=int, default=8)
     ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out"))
     ap.add_argument("--params", default=None, help="evaluate these parameters (.npy) instead of training")
+    ap.add_argument("--synthetic", type=int, default=0,
+                    help="train on this many random fields for --epochs, no DSMC and no evaluation "
+                         "(kernel timing of the training step)")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
     net = S.Net(edge=S.EDGE, max_batch=args.batch, seed=1)
     losses, t_gen, t_train, nfields = [], 0.0, 0.0, 0
+    if args.synthetic:
+        rng = np.random.default_rng(0)
+        cells = (S.EDGE // 4) ** 3
+        net.set_data((rng.random((args.synthetic, S.EDGE ** 3)) < 0.4).astype(np.uint8),
+                     rng.normal(size=(args.synthetic, 2)).astype(np.float32),
+                     rng.normal(size=(args.synthetic, cells, 5)).astype(np.float32),
+                     np.ones((args.synthetic, cells), np.float32))
+        t1 = time.time()
+        losses = net.train(args.epochs, args.batch, args.lr, seed=0)
+        n_batches = args.epochs * (args.synthetic // args.batch)
+        log(f"[surrogate] {n_batches} training batches of {args.batch} in {time.time() - t1:.2f}s, "
+            f"losses {losses}")
+        return
     if args.params:
         net.set_params(np.load(args.params).astype(np.float32))
         args.epochs = 0
