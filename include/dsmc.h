@@ -172,7 +172,8 @@ dsmc_status dsmc_get_kernel_times(dsmc_handle *h, int *nkernels, char *names, in
  * parameters).  Both knobs leave results bit-identical. */
 enum {
     DSMC_OPT_COLLIDE_CHUNK = 7, /* cells per collide work-queue entry, 1..64 (0: automatic) */
-    DSMC_OPT_GRAPH_STEPS = 8    /* steps per captured graph, even, 2..64 (default 2) */
+    DSMC_OPT_GRAPH_STEPS = 8,   /* steps per captured graph, even, 2..64 (default 2) */
+    DSMC_OPT_FUSED = 9          /* 1 (default): gather, collide and sampling in one pass */
 };
 typedef struct {
     int graph_steps;            /* chosen steps per captured graph */

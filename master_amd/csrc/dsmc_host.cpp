@@ -75,6 +75,7 @@ struct dsmc_handle {
     uint32_t lds_cap = 0;
     bool lds_auto = true;
     uint32_t collide_chunk = 0;     // 0: automatic
+    bool fused = true;              // gather and collide in one pass
     int graph_steps = 2;            // steps per captured graph (even)
     // host mirrors of System counters
     int64_t steps = 0;
@@ -530,16 +531,23 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
         Scope sc(h, "k_bucket_cells");
         HIPCHK(h, launch_bucket_cells(bucket_args(h), s));
     }
-    {
-        Scope sc(h, "k_gather");
-        HIPCHK(h, launch_gather(gather_args(h, false, false), h->ncells, s));
-    }
-    h->cur = 1 - h->cur;
-    h->scur = 1 - h->scur;
     const bool collide = (h->phases & 4) != 0;
-    if (collide || sample) {
-        Scope sc(h, "k_collide");
-        HIPCHK(h, launch_collide(collide_args(h, collide, sample), h->ncells, h->collide_chunk, s));
+    if (h->fused && (collide || sample)) {
+        Scope sc(h, "k_gather_collide");
+        HIPCHK(h, launch_gather_collide(gather_args(h, collide, sample), h->ncells, s));
+        h->cur = 1 - h->cur;
+        h->scur = 1 - h->scur;
+    } else {
+        {
+            Scope sc(h, "k_gather");
+            HIPCHK(h, launch_gather(gather_args(h, false, false), h->ncells, s));
+        }
+        h->cur = 1 - h->cur;
+        h->scur = 1 - h->scur;
+        if (collide || sample) {
+            Scope sc(h, "k_collide");
+            HIPCHK(h, launch_collide(collide_args(h, collide, sample), h->ncells, h->collide_chunk, s));
+        }
     }
     if (h->totals) {
         int b = h->cur;
@@ -1144,6 +1152,10 @@ dsmc_status dsmc_set_option(dsmc_handle *h, int option, int64_t value) {
     case DSMC_OPT_COLLIDE_CHUNK:
         if (value < 0 || value > 64) return DSMC_ERR_INVALID_ARGUMENT;
         h->collide_chunk = (uint32_t)value;
+        drop_graphs(h);
+        return DSMC_OK;
+    case DSMC_OPT_FUSED:
+        h->fused = value != 0;
         drop_graphs(h);
         return DSMC_OK;
     case DSMC_OPT_GRAPH_STEPS:

@@ -123,6 +123,8 @@ hipError_t launch_scan(const ScanArgs &a, hipStream_t s);
 hipError_t launch_bucket(const BucketArgs &a, hipStream_t s);
 hipError_t launch_bucket_cells(const BucketArgs &a, hipStream_t s);
 hipError_t launch_gather(const GatherArgs &a, uint32_t ncells, hipStream_t s);
+// gather, NTC collisions and sampling in one pass (do_collide / do_sample)
+hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, hipStream_t s);
 // chunk: cells per work-queue entry (0: automatic)
 hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, uint32_t chunk, hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);

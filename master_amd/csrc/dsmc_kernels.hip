@@ -699,6 +699,201 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ gather + collide
+// The stable gather of k_gather and the collide of k_collide in one wave per
+// destination cell: positions and voxels go to the other state buffer as in
+// k_gather, velocities into LDS at their final place in the cell, where the
+// NTC loop (ntc_cell) and the FP64 moments run before the velocities are
+// written once.  This saves the collide pass's velocity re-read and write
+// and overlaps the NTC loop of some waves with the gather traffic of others.
+// Results equal k_gather followed by k_collide bit for bit.  Cells larger
+// than the LDS capacity write their velocities to the new buffer and collide
+// there (coherent global path).
+__global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
+    const uint32_t c = blockIdx.x + a.cell_begin;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t K = a.lds_cap;
+    float *lx = dyn_lds, *ly = dyn_lds + K, *lz = dyn_lds + 2 * K;
+    uint32_t *mark = (uint32_t *)(dyn_lds + 3 * K);
+    __shared__ uint32_t keys[GATHER_KEYS];
+    const uint32_t ns = a.start_new[c], ne = a.start_new[c + 1];
+    const uint32_t n = ne - ns;
+    const double coeff = a.coeff[c];
+    const bool active = coeff > 0.0;
+    if (n == 0 && !(active && a.do_collide)) return;
+    const uint32_t os = a.start[c], oe = a.start[c + 1];
+    const uint32_t bs = a.bstart[c], be = a.bstart[c + 1];
+    const uint32_t nb = be - bs;
+    const uint32_t n_stay = n - nb;
+    const Member M = a.members[a.cell_member[c]];
+    const bool in_lds = n <= K;
+    for (uint32_t i = lane; i < MARKS; i += WAVE) mark[i] = 0xffffffffu;
+
+    // ---- gather (k_gather), velocities to LDS
+    uint32_t n_low = 0;
+    const bool keys_lds = nb > WAVE && nb <= GATHER_KEYS;
+    for (uint32_t i = lane; i < nb; i += WAVE) {
+        const uint32_t k = a.bucket[bs + i];
+        if (keys_lds) keys[i] = k;
+        n_low += k < os ? 1u : 0u;
+    }
+    n_low = wave_sum(n_low);
+    if (keys_lds) __syncthreads();
+    uint32_t run = 0;
+    for (uint32_t base = os; base < oe; base += GATHER_BATCH * WAVE) {
+        float ox[GATHER_BATCH], oy[GATHER_BATCH], oz[GATHER_BATCH], vx[GATHER_BATCH], vy[GATHER_BATCH], vz[GATHER_BATCH];
+        uint32_t vp[GATHER_BATCH];
+#pragma unroll
+        for (int q = 0; q < GATHER_BATCH; q++) {
+            const uint32_t p = base + q * WAVE + lane;
+            vp[q] = 0xffffffffu;
+            if (p < oe) {
+                vp[q] = a.vox[p];
+                ox[q] = a.ox[p]; oy[q] = a.oy[p]; oz[q] = a.oz[p];
+                vx[q] = a.vx[p]; vy[q] = a.vy[p]; vz[q] = a.vz[p];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < GATHER_BATCH; q++) {
+            const uint32_t p = base + q * WAVE + lane;
+            bool stay = false;
+            if (p < oe) {
+                int ix, iy, iz;
+                unpack_vox(vp[q], ix, iy, iz);
+                stay = cell_of(M, ix, iy, iz) == c;
+            }
+            const uint64_t sm = __ballot(stay);
+            if (stay) {
+                const uint32_t L = n_low + run + mbcnt(sm);
+                const uint32_t d = ns + L;
+                a.ox2[d] = ox[q]; a.oy2[d] = oy[q]; a.oz2[d] = oz[q];
+                a.vox2[d] = vp[q];
+                if (in_lds) { lx[L] = vx[q]; ly[L] = vy[q]; lz[L] = vz[q]; }
+                else { a.vx2[d] = vx[q]; a.vy2[d] = vy[q]; a.vz2[d] = vz[q]; }
+                if (a.rec_perm) a.rec_perm[d] = (int32_t)p;
+            }
+            run += (uint32_t)__popcll(sm);
+        }
+    }
+    for (uint32_t base = 0; base < nb; base += WAVE) {
+        const uint32_t i = base + lane;
+        const bool valid = i < nb;
+        const uint32_t key = valid ? a.bucket[bs + i] : 0xffffffffu;
+        uint32_t rank = 0;
+        if (nb <= WAVE) {
+            for (uint32_t j = 0; j < nb; j++) {
+                const uint32_t kj = (uint32_t)__shfl((int)key, (int)j, WAVE);
+                rank += kj < key ? 1u : 0u;
+            }
+        } else if (keys_lds) {
+            for (uint32_t j = 0; j < nb; j++) rank += keys[j] < key ? 1u : 0u;
+        } else {
+            for (uint32_t j = 0; j < nb; j++) rank += a.bucket[bs + j] < key ? 1u : 0u;
+        }
+        if (valid) {
+            const uint32_t L = rank < n_low ? rank : n_stay + rank;
+            const uint32_t d = ns + L;
+            const uint32_t q = key;
+            a.ox2[d] = a.ox[q]; a.oy2[d] = a.oy[q]; a.oz2[d] = a.oz[q];
+            a.vox2[d] = a.vox[q];
+            const float x = a.vx[q], y = a.vy[q], z = a.vz[q];
+            if (in_lds) { lx[L] = x; ly[L] = y; lz[L] = z; }
+            else { a.vx2[d] = x; a.vy2[d] = y; a.vz2[d] = z; }
+            if (a.rec_perm) a.rec_perm[d] = (int32_t)q;
+        }
+    }
+    if (in_lds) {
+        __syncthreads();
+    } else {
+        // the global path reads other lanes' stores: make them visible
+        __threadfence();
+        __syncthreads();
+    }
+
+    // ---- collide (k_collide)
+    if (active && a.do_collide) {
+        const uint32_t step = *a.step;
+        const double vrm = a.vr_max[c];
+        const double select = coeff * (double)n * (double)((int)n - 1) * vrm + a.rest[c];
+        const int pairs = (int)round(select);
+        double crm = vrm;
+        uint32_t acc = 0;
+        if (n >= 2) {
+            if (in_lds) {
+                LdsStore S{lx, ly, lz};
+                ntc_cell(M.key_lo, M.key_hi, c - M.cell0, step, n, pairs, vrm, S, mark, crm, acc);
+            } else {
+                GlobalStore S{a.vx2 + ns, a.vy2 + ns, a.vz2 + ns};
+                ntc_cell(M.key_lo, M.key_hi, c - M.cell0, step, n, pairs, vrm, S, mark, crm, acc);
+            }
+        }
+        // the step's totals come from stat_pairs / stat_acc (k_cell_totals):
+        // one atomic per cell on three counters would serialise
+        if (lane == 0) {
+            a.rest[c] = select - (double)pairs;
+            if (n >= 2) a.vr_max[c] = crm;
+            a.stat_pairs[c] = pairs;
+            a.stat_acc[c] = (int32_t)acc;
+        }
+    }
+    if (n == 0) return;
+    // ---- write-back and per-cell FP64 moments (paths apart, as in k_collide)
+    double mm[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (in_lds) {
+        for (uint32_t i = lane; i < n; i += WAVE) {
+            const float x = lx[i], y = ly[i], z = lz[i];
+            a.vx2[ns + i] = x; a.vy2[ns + i] = y; a.vz2[ns + i] = z;
+            const double dx = x, dy = y, dz = z;
+            mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
+            mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+        }
+    } else {
+        for (uint32_t i = lane; i < n; i += WAVE) {
+            const double dx = GlobalStore::ldf(a.vx2 + ns + i), dy = GlobalStore::ldf(a.vy2 + ns + i),
+                         dz = GlobalStore::ldf(a.vz2 + ns + i);
+            mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
+            mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+        }
+    }
+    if (a.do_sample && active) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) mm[q] = wave_sum(mm[q]);
+        double mine = mm[0];
+#pragma unroll
+        for (int q = 1; q < 7; q++) mine = lane == (uint32_t)q ? mm[q] : mine;
+        if (lane < 7) unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + c, mine);
+    }
+}
+
+// Collision, candidate and overflow totals of a fused gather + collide pass
+// from the per-cell statistics: block sums, one atomic per block.
+__global__ __launch_bounds__(256) void k_cell_totals(const int32_t *pairs, const int32_t *accepted,
+                                                     const uint32_t *start, const double *coeff, uint32_t ncells,
+                                                     uint32_t lds_cap, unsigned long long *counters) {
+    __shared__ unsigned long long red[3][4];
+    unsigned long long acc = 0, cand = 0, over = 0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+        if (!(coeff[c] > 0.0)) continue;
+        const uint32_t n = start[c + 1] - start[c];
+        acc += (unsigned long long)accepted[c];
+        if (n >= 2 && pairs[c] > 0) cand += (unsigned long long)pairs[c];
+        over += n > lds_cap ? 1ull : 0ull;
+    }
+    acc = wave_sum(acc);
+    cand = wave_sum(cand);
+    over = wave_sum(over);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = acc; red[1][w] = cand; red[2][w] = over; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t0 = 0, t1 = 0, t2 = 0;
+        for (int k = 0; k < 4; k++) { t0 += red[0][k]; t1 += red[1][k]; t2 += red[2][k]; }
+        if (t0) atomicAdd(&counters[CNT_COLL], t0);
+        if (t1) atomicAdd(&counters[CNT_CAND], t1);
+        if (t2) atomicAdd(&counters[CNT_OVERFLOW], t2);
+    }
+}
+
 // ------------------------------------------------------------------ seeding
 __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += gridDim.x * blockDim.x) {
@@ -864,6 +1059,19 @@ hipError_t launch_gather(const GatherArgs &a, uint32_t ncells, hipStream_t s) {
     if (ncells == 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather, dim3(ncells), dim3(64), 0, s, a);
     LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, hipStream_t s) {
+    if (ncells == 0) return hipSuccess;
+    const size_t lds = (size_t)a.lds_cap * 12 + MARKS * 4;
+    hipLaunchKernelGGL(k_gather_collide, dim3(ncells), dim3(64), lds, s, a);
+    LAUNCH_CHECK();
+    if (a.do_collide) {
+        hipLaunchKernelGGL(k_cell_totals, dim3(256), dim3(256), 0, s, a.stat_pairs, a.stat_acc, a.start_new,
+                           a.coeff, a.ncells, a.lds_cap, a.counters);
+        LAUNCH_CHECK();
+    }
     return hipSuccess;
 }
 

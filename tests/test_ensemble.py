@@ -64,8 +64,9 @@ def test_ensemble_member_equals_solo_run(gpu, graph):
 
 @pytest.mark.gpu
 def test_tuning_knobs_leave_results_unchanged(gpu):
-    """Collide chunking, graph length and the autotuner's choice change
-    launch shapes only: trajectories stay bit-identical."""
+    """Collide chunking, graph length, the fused gather + collide pass and the
+    autotuner's choice change launch shapes only: trajectories stay
+    bit-identical."""
     dsmc = gpu
     members = [_member(dsmc, s) for s in SPECS[:2]]
 
@@ -85,7 +86,8 @@ def test_tuning_knobs_leave_results_unchanged(gpu):
     ref = (tuned.state(), tuned.moments()[0], tuned.counters()["collisions"])
     tuned.close()
     for knobs in ({}, {dsmc.OPT_COLLIDE_CHUNK: 1, dsmc.OPT_GRAPH_STEPS: 4},
-                  {dsmc.OPT_COLLIDE_CHUNK: 16, dsmc.OPT_GRAPH_STEPS: 8}):
+                  {dsmc.OPT_COLLIDE_CHUNK: 16, dsmc.OPT_GRAPH_STEPS: 8},
+                  {dsmc.OPT_FUSED: 0}, {dsmc.OPT_FUSED: 0, dsmc.OPT_USE_GRAPH: 0}):
         g = make()
         for k, v in knobs.items():
             g.set_option(k, v)
