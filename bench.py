@@ -16,8 +16,10 @@ Prints one JSON line (the driver contract) with
   mean launch time (HIP events on the library's stream), against the 8 TB/s
   HBM peak.  Algorithmic bytes per particle: k_move 56 (28-byte state read
   and written), k_gather 56 (state read from the old order, written in cell
-  order), k_collide 24 (velocities read and written); the step moves 140
-  (sort row 60 = gather 56 + the 4-byte cell key, read inside k_move).
+  order), k_collide 24 (velocities read and written), k_gather_collide 56
+  (the fused gather + collide pass the library runs by default: the collide
+  adds no traffic); the step's reference count is 140 (sort row 60 = gather
+  56 + the 4-byte cell key, read inside k_move; collide 24).
   `traffic` is the HBM bytes one launch of that kernel really moves, from two
   rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over the same workload,
   each scaled by what the counter reads for a plain 1 GiB copy in the same
@@ -43,7 +45,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per particle and launch
-KERNEL_BYTES = {"k_move": 56, "k_gather": 56, "k_collide": 24}
+KERNEL_BYTES = {"k_move": 56, "k_gather": 56, "k_collide": 24, "k_gather_collide": 56}
 STEP_BYTES = 140
 COPY_BYTES = 1 << 30
 

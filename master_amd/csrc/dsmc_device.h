@@ -108,6 +108,13 @@ struct VoxelView {
     int tx0, ty0, tz0;        // tile origin (unwrapped voxel coords)
     int TX, TY, TZ;           // tile extents
     int n0, n1, n2;
+    // the tile only; V_WALL outside it
+    __device__ __forceinline__ uint32_t tile_at(int ix, int iy, int iz) const {
+        int a = ix - tx0, b = iy - ty0, c = iz - tz0;
+        if ((unsigned)a < (unsigned)TX && (unsigned)b < (unsigned)TY && (unsigned)c < (unsigned)TZ)
+            return tile[(a * TY + b) * TZ + c];
+        return V_WALL;
+    }
     __device__ __forceinline__ uint32_t at(int ix, int iy, int iz) const {
         int a = ix - tx0, b = iy - ty0, c = iz - tz0;
         if ((unsigned)a < (unsigned)TX && (unsigned)b < (unsigned)TY && (unsigned)c < (unsigned)TZ)
@@ -243,12 +250,16 @@ __device__ __forceinline__ void wall_collide(const Member &M, const float *frame
 // Coordinates stay unwrapped during the flight (the reference's halo grid)
 // and are wrapped once at the end.  Returns the last wall-hit voxel's linear
 // id or -1.
-__device__ __forceinline__ int32_t move_particle(const Member &M, const VoxelView &V,
-                                                 const float *frames, uint32_t step,
-                                                 uint32_t pidx, int &ix, int &iy, int &iz,
-                                                 float &ox, float &oy, float &oz, float &vx,
-                                                 float &vy, float &vz, MoveCounters &cnt,
-                                                 uint32_t &nhit) {
+__device__ __forceinline__ int32_t move_general(const Member &M, const VoxelView &V,
+                                                const float *frames, uint32_t step,
+                                                uint32_t pidx, int &ix, int &iy, int &iz,
+                                                float &ox, float &oy, float &oz, float &vx,
+                                                float &vy, float &vz, MoveCounters &cnt,
+                                                uint32_t &nhit) {
+    // keep the wall and reseed draws' Philox rounds inside this path: the
+    // compiler would otherwise hoist their particle-dependent first rounds
+    // into every particle's move
+    asm volatile("" : "+v"(pidx));
     // System::accelerate: v[flow_dir] += gravity*dt; adding an exact 0 to the
     // other components leaves them unchanged
     vx = vx + M.acc[0];
@@ -356,6 +367,42 @@ __device__ __forceinline__ int32_t move_particle(const Member &M, const VoxelVie
     }
     nhit = hit;
     return last;
+}
+
+// The common case first: start and landing voxel both empty voxels of the
+// cell's tile.  Its arithmetic is move_general's for that case, operation for
+// operation (accelerate, w = v / h, advance1 per axis, one periodic wrap), so
+// both give bit-identical results; any other particle takes move_general
+// from its original inputs.
+__device__ __forceinline__ int32_t move_particle(const Member &M, const VoxelView &V,
+                                                 const float *frames, uint32_t step,
+                                                 uint32_t pidx, int &ix, int &iy, int &iz,
+                                                 float &ox, float &oy, float &oz, float &vx,
+                                                 float &vy, float &vz, MoveCounters &cnt,
+                                                 uint32_t &nhit) {
+    const float ax = vx + M.acc[0], ay = vy + M.acc[1], az = vz + M.acc[2];
+    int jx = ix, jy = iy, jz = iz;
+    float px = ox, py = oy, pz = oz;
+    bool fast = V.tile_at(ix, iy, iz) == V_EMPTY;
+    if (fast) {
+        const float dt = M.dt;
+        advance1(jx, px, ax * M.inv_h[0], dt);
+        advance1(jy, py, ay * M.inv_h[1], dt);
+        advance1(jz, pz, az * M.inv_h[2], dt);
+        fast = V.tile_at(jx, jy, jz) == V_EMPTY;
+    }
+    if (fast) {
+        // tile voxels lie within one voxel of the box: one wrap suffices
+        if (jx >= M.n[0]) { jx -= M.n[0]; cnt.periodic[0]++; } else if (jx < 0) { jx += M.n[0]; cnt.periodic[0]--; }
+        if (jy >= M.n[1]) { jy -= M.n[1]; cnt.periodic[1]++; } else if (jy < 0) { jy += M.n[1]; cnt.periodic[1]--; }
+        if (jz >= M.n[2]) { jz -= M.n[2]; cnt.periodic[2]++; } else if (jz < 0) { jz += M.n[2]; cnt.periodic[2]--; }
+        ix = jx; iy = jy; iz = jz;
+        ox = px; oy = py; oz = pz;
+        vx = ax; vy = ay; vz = az;
+        nhit = 0;
+        return -1;
+    }
+    return move_general(M, V, frames, step, pidx, ix, iy, iz, ox, oy, oz, vx, vy, vz, cnt, nhit);
 }
 
 // Wave-level helpers (wave64).

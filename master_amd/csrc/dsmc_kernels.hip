@@ -49,7 +49,10 @@ __device__ __forceinline__ void load_tile(const Member &M, const uint8_t *types,
 }
 
 // ------------------------------------------------------------------ move
-__global__ __launch_bounds__(64) void k_move(MoveArgs a) {
+#ifndef DSMC_MOVE_WAVES_PER_EU
+#define DSMC_MOVE_WAVES_PER_EU 1
+#endif
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WAVES_PER_EU, 8))) void k_move(MoveArgs a) {
     const uint32_t c = blockIdx.x + a.cell_begin;
     const uint32_t lane = threadIdx.x;
     const uint32_t s0 = a.start[c], s1 = a.start[c + 1];
@@ -96,11 +99,18 @@ __global__ __launch_bounds__(64) void k_move(MoveArgs a) {
             uint32_t nhit = 0;
             // draws are keyed by the index inside the member, so a member of an
             // ensemble steps exactly as it would alone
+            const uint32_t v0x = __float_as_uint(vx), v0y = __float_as_uint(vy), v0z = __float_as_uint(vz);
             int32_t last = move_particle(M, V, frames, step, p - M.pfirst, ix, iy, iz, ox, oy, oz, vx, vy, vz,
                                          cnt, nhit);
+            // unchanged words are not stored: off the body-force axis a
+            // velocity changes only at a wall, a voxel index only when the
+            // particle crosses a face, so most of their cache lines stay clean
             a.ox[p] = ox; a.oy[p] = oy; a.oz[p] = oz;
-            a.vx[p] = vx; a.vy[p] = vy; a.vz[p] = vz;
-            a.vox[p] = pack_vox(ix, iy, iz);
+            if (__float_as_uint(vx) != v0x) a.vx[p] = vx;
+            if (__float_as_uint(vy) != v0y) a.vy[p] = vy;
+            if (__float_as_uint(vz) != v0z) a.vz[p] = vz;
+            const uint32_t vpn = pack_vox(ix, iy, iz);
+            if (vpn != vp) a.vox[p] = vpn;
             newc = cell_of(M, ix, iy, iz);
             if (a.rec_nhit) {
                 a.rec_nhit[p] = (int32_t)nhit;
