@@ -725,7 +725,8 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
     const uint32_t K = a.lds_cap;
     float *lx = dyn_lds, *ly = dyn_lds + K, *lz = dyn_lds + 2 * K;
     uint32_t *mark = (uint32_t *)(dyn_lds + 3 * K);
-    __shared__ uint32_t keys[GATHER_KEYS];
+    // the mark table is free until the NTC loop: it holds the arrival keys
+    uint32_t *keys = mark;
     const uint32_t ns = a.start_new[c], ne = a.start_new[c + 1];
     const uint32_t n = ne - ns;
     const double coeff = a.coeff[c];
@@ -737,11 +738,10 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
     const uint32_t n_stay = n - nb;
     const Member M = a.members[a.cell_member[c]];
     const bool in_lds = n <= K;
-    for (uint32_t i = lane; i < MARKS; i += WAVE) mark[i] = 0xffffffffu;
 
     // ---- gather (k_gather), velocities to LDS
     uint32_t n_low = 0;
-    const bool keys_lds = nb > WAVE && nb <= GATHER_KEYS;
+    const bool keys_lds = nb > WAVE && nb <= MARKS;
     for (uint32_t i = lane; i < nb; i += WAVE) {
         const uint32_t k = a.bucket[bs + i];
         if (keys_lds) keys[i] = k;
@@ -819,6 +819,8 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
         __threadfence();
         __syncthreads();
     }
+    for (uint32_t i = lane; i < MARKS; i += WAVE) mark[i] = 0xffffffffu;
+    __syncthreads();
 
     // ---- collide (k_collide)
     if (active && a.do_collide) {
