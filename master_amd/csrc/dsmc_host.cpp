@@ -398,6 +398,7 @@ GatherArgs gather_args(dsmc_handle *h, bool collide, bool sample) {
     g.step = h->d_step;
     g.counters = h->d_cnt;
     g.rec_perm = h->record ? h->d_rec_perm : nullptr;
+    g.queue = h->d_mvcount + 1;
     g.ncells = h->ncells;
     g.cell_begin = 0;
     g.lds_cap = h->lds_cap;
@@ -534,7 +535,7 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
     const bool collide = (h->phases & 4) != 0;
     if (h->fused && (collide || sample)) {
         Scope sc(h, "k_gather_collide");
-        HIPCHK(h, launch_gather_collide(gather_args(h, collide, sample), h->ncells, s));
+        HIPCHK(h, launch_gather_collide(gather_args(h, collide, sample), h->ncells, h->collide_chunk, s));
         h->cur = 1 - h->cur;
         h->scur = 1 - h->scur;
     } else {

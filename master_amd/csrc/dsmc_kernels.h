@@ -69,7 +69,9 @@ struct GatherArgs {
     const uint32_t *step;
     unsigned long long *counters;
     int32_t *rec_perm;
+    uint32_t *queue;                    // work queue of the fused pass, zeroed by k_begin_step
     uint32_t ncells, cell_begin, lds_cap;
+    uint32_t chunk;                     // cells per queue entry (set by launch_gather_collide)
     int do_collide, do_sample;
 };
 
@@ -124,7 +126,7 @@ hipError_t launch_bucket(const BucketArgs &a, hipStream_t s);
 hipError_t launch_bucket_cells(const BucketArgs &a, hipStream_t s);
 hipError_t launch_gather(const GatherArgs &a, uint32_t ncells, hipStream_t s);
 // gather, NTC collisions and sampling in one pass (do_collide / do_sample)
-hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, hipStream_t s);
+hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t chunk, hipStream_t s);
 // chunk: cells per work-queue entry (0: automatic)
 hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, uint32_t chunk, hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);

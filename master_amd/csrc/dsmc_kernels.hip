@@ -40,11 +40,25 @@ __device__ __forceinline__ void load_tile(const Member &M, const uint8_t *types,
     V.TY = M.b[1] + 2;
     V.TZ = M.b[2] + 2;
     V.n0 = M.n[0]; V.n1 = M.n[1]; V.n2 = M.n[2];
-    int tot = V.TX * V.TY * V.TZ;
-    for (int i = (int)threadIdx.x; i < tot; i += WAVE) {
-        int c = i % V.TZ, b = (i / V.TZ) % V.TY, a = i / (V.TY * V.TZ);
-        int x = wrapi(V.tx0 + a, M.n[0]), y = wrapi(V.ty0 + b, M.n[1]), z = wrapi(V.tz0 + c, M.n[2]);
-        tile[i] = types[((size_t)x * M.n[1] + y) * M.n[2] + z];
+    const uint32_t TY = (uint32_t)V.TY, TZ = (uint32_t)V.TZ, TYZ = TY * TZ;
+    const uint32_t tot = (uint32_t)V.TX * TYZ;
+    // tile coordinates of voxel i = lane, then stepped by WAVE with carries:
+    // no integer division per voxel
+    uint32_t i = threadIdx.x;
+    uint32_t a = i / TYZ, b = (i - a * TYZ) / TZ, c = i - a * TYZ - b * TZ;
+    const uint32_t da = WAVE / TYZ, db = (WAVE - da * TYZ) / TZ, dc = WAVE - da * TYZ - db * TZ;
+    const uint32_t n1 = (uint32_t)M.n[1], n2 = (uint32_t)M.n[2];
+    for (; i < tot; i += WAVE) {
+        const uint32_t x = (uint32_t)wrapi(V.tx0 + (int)a, M.n[0]);
+        const uint32_t y = (uint32_t)wrapi(V.ty0 + (int)b, M.n[1]);
+        const uint32_t z = (uint32_t)wrapi(V.tz0 + (int)c, M.n[2]);
+        // axes of at most 1024 voxels: 24-bit products, index below 2^30
+        tile[i] = types[__umul24(__umul24(x, n1) + y, n2) + z];
+        c += dc;
+        b += db;
+        a += da;
+        if (c >= TZ) { c -= TZ; b++; }
+        if (b >= TY) { b -= TY; a++; }
     }
 }
 
@@ -56,23 +70,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
     const uint32_t c = blockIdx.x + a.cell_begin;
     const uint32_t lane = threadIdx.x;
     const uint32_t s0 = a.start[c], s1 = a.start[c + 1];
+    const uint32_t mi = a.cell_member[c];
     if (s0 == s1) {
         if (lane == 0) a.leave[c] = 0;
         return;
     }
-    const Member M = a.members[a.cell_member[c]];
-    const uint8_t *types = a.types + M.vox0;
-    const float *frames = a.frames + 9 * M.vox0;
-    __shared__ uint8_t tile[MAX_TILE];
-    int cx, cy, cz;
-    cell_coords(M, c - M.cell0, cx, cy, cz);
-    VoxelView V;
-    load_tile(M, types, tile, cx, cy, cz, V);
-    __syncthreads();
-    const uint32_t step = *a.step;
-    MoveCounters cnt = {0, 0, 0, {0, 0, 0}};
-    uint32_t leavers = 0;
-    // software pipeline: the next chunk's loads are in flight while this
+    // software pipeline: the first chunk's loads go out before the member
+    // descriptor and the voxel tile are fetched, the next chunk's while this
     // chunk is moved
     uint32_t p = s0 + lane;
     float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
@@ -82,6 +86,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
         vx = a.vx[p]; vy = a.vy[p]; vz = a.vz[p];
         vp = a.vox[p];
     }
+    const Member M = a.members[mi];
+    const uint8_t *types = a.types + M.vox0;
+    const float *frames = a.frames + 9 * M.vox0;
+    __shared__ uint8_t tile[MAX_TILE];
+    int cx, cy, cz;
+    cell_coords(M, c - M.cell0, cx, cy, cz);
+    VoxelView V;
+    load_tile(M, types, tile, cx, cy, cz, V);
+    // one wave per block, and a wave's LDS accesses complete in order: the
+    // tile is visible to every lane without a barrier
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const uint32_t step = *a.step;
+    MoveCounters cnt = {0, 0, 0, {0, 0, 0}};
+    uint32_t leavers = 0;
     for (uint32_t base = s0; base < s1; base += WAVE) {
         const uint32_t pn = p + WAVE;
         float nox = 0, noy = 0, noz = 0, nvx = 0, nvy = 0, nvz = 0;
@@ -719,24 +737,27 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
 // Results equal k_gather followed by k_collide bit for bit.  Cells larger
 // than the LDS capacity write their velocities to the new buffer and collide
 // there (coherent global path).
-__global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
-    const uint32_t c = blockIdx.x + a.cell_begin;
+// Per-cell inputs of the fused pass, read from a chunk's metadata.
+struct GcMeta {
+    uint32_t ns, ne, os, oe, bs, be;
+    double coeff, vrm, rest;
+};
+
+__device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_t c, const GcMeta &m,
+                                                    const Member &M, float *lx, float *ly, float *lz,
+                                                    uint32_t *mark, uint32_t K) {
     const uint32_t lane = threadIdx.x;
-    const uint32_t K = a.lds_cap;
-    float *lx = dyn_lds, *ly = dyn_lds + K, *lz = dyn_lds + 2 * K;
-    uint32_t *mark = (uint32_t *)(dyn_lds + 3 * K);
     // the mark table is free until the NTC loop: it holds the arrival keys
     uint32_t *keys = mark;
-    const uint32_t ns = a.start_new[c], ne = a.start_new[c + 1];
+    const uint32_t ns = m.ns, ne = m.ne;
     const uint32_t n = ne - ns;
-    const double coeff = a.coeff[c];
+    const double coeff = m.coeff;
     const bool active = coeff > 0.0;
     if (n == 0 && !(active && a.do_collide)) return;
-    const uint32_t os = a.start[c], oe = a.start[c + 1];
-    const uint32_t bs = a.bstart[c], be = a.bstart[c + 1];
+    const uint32_t os = m.os, oe = m.oe;
+    const uint32_t bs = m.bs, be = m.be;
     const uint32_t nb = be - bs;
     const uint32_t n_stay = n - nb;
-    const Member M = a.members[a.cell_member[c]];
     const bool in_lds = n <= K;
 
     // ---- gather (k_gather), velocities to LDS
@@ -825,8 +846,8 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
     // ---- collide (k_collide)
     if (active && a.do_collide) {
         const uint32_t step = *a.step;
-        const double vrm = a.vr_max[c];
-        const double select = coeff * (double)n * (double)((int)n - 1) * vrm + a.rest[c];
+        const double vrm = m.vrm;
+        const double select = coeff * (double)n * (double)((int)n - 1) * vrm + m.rest;
         const int pairs = (int)round(select);
         double crm = vrm;
         uint32_t acc = 0;
@@ -874,6 +895,72 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
 #pragma unroll
         for (int q = 1; q < 7; q++) mine = lane == (uint32_t)q ? mm[q] : mine;
         if (lane < 7) unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + c, mine);
+    }
+}
+
+// The fused pass as persistent waves on the work queue (as k_collide): the
+// metadata of a chunk of cells arrives one cell per lane, one chunk ahead,
+// so no cell starts with a chain of dependent metadata loads.
+struct GcBatch {
+    uint32_t ns, ne, os, oe, bs, be, member;
+    double coeff, vrm, rest;
+};
+
+__device__ __forceinline__ GcBatch load_gc_batch(const GatherArgs &a, uint32_t chunk) {
+    GcBatch b{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0.0, 0.0, 0.0};
+    const uint64_t c = (uint64_t)chunk * a.chunk + threadIdx.x;
+    if (threadIdx.x < a.chunk && c < a.ncells) {
+        b.ns = a.start_new[c];
+        b.ne = a.start_new[c + 1];
+        b.os = a.start[c];
+        b.oe = a.start[c + 1];
+        b.bs = a.bstart[c];
+        b.be = a.bstart[c + 1];
+        b.member = a.cell_member[c];
+        b.coeff = a.coeff[c];
+        b.vrm = a.vr_max[c];
+        b.rest = a.rest[c];
+    }
+    return b;
+}
+
+__global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
+    const uint32_t K = a.lds_cap;
+    float *lx = dyn_lds, *ly = dyn_lds + K, *lz = dyn_lds + 2 * K;
+    uint32_t *mark = (uint32_t *)(dyn_lds + 3 * K);
+    uint32_t q0 = grab_chunk(a.queue);
+    GcBatch cur = load_gc_batch(a, q0);
+    uint32_t q1 = grab_chunk(a.queue);
+    GcBatch nxt = load_gc_batch(a, q1);
+    uint32_t j = 0, member = 0xffffffffu;
+    Member M;
+    for (;;) {
+        const uint64_t c64 = (uint64_t)q0 * a.chunk + j;
+        if (c64 >= a.ncells) break;
+        const uint32_t c = (uint32_t)c64;
+        GcMeta m;
+        m.ns = __builtin_amdgcn_readlane(cur.ns, j);
+        m.ne = __builtin_amdgcn_readlane(cur.ne, j);
+        m.os = __builtin_amdgcn_readlane(cur.os, j);
+        m.oe = __builtin_amdgcn_readlane(cur.oe, j);
+        m.bs = __builtin_amdgcn_readlane(cur.bs, j);
+        m.be = __builtin_amdgcn_readlane(cur.be, j);
+        m.coeff = readlane_d(cur.coeff, j);
+        m.vrm = readlane_d(cur.vrm, j);
+        m.rest = readlane_d(cur.rest, j);
+        const uint32_t mi = __builtin_amdgcn_readlane(cur.member, j);
+        if (mi != member) {            // members change only at member boundaries
+            M = a.members[mi];
+            member = mi;
+        }
+        gather_collide_cell(a, c, m, M, lx, ly, lz, mark, K);
+        if (++j == a.chunk) {
+            cur = nxt;
+            q0 = q1;
+            q1 = grab_chunk(a.queue);
+            nxt = load_gc_batch(a, q1);
+            j = 0;
+        }
     }
 }
 
@@ -1074,10 +1161,24 @@ hipError_t launch_gather(const GatherArgs &a, uint32_t ncells, hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, hipStream_t s) {
+hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t chunk_opt, hipStream_t s) {
     if (ncells == 0) return hipSuccess;
     const size_t lds = (size_t)a.lds_cap * 12 + MARKS * 4;
-    hipLaunchKernelGGL(k_gather_collide, dim3(ncells), dim3(64), lds, s, a);
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_collide, 64, lds);
+    if (e != hipSuccess) return e;
+    uint32_t grid = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
+    GatherArgs b = a;
+    uint32_t chunk = chunk_opt ? chunk_opt : ncells / (grid * 8u);
+    b.chunk = chunk < 1u ? 1u : (chunk > 64u ? 64u : chunk);
+    if (!chunk_opt && b.chunk > 8u) b.chunk = 8u;
+    const uint32_t nchunks = (ncells + b.chunk - 1) / b.chunk;
+    if (grid > nchunks) grid = nchunks;
+    hipLaunchKernelGGL(k_gather_collide, dim3(grid), dim3(64), lds, s, b);
     LAUNCH_CHECK();
     if (a.do_collide) {
         hipLaunchKernelGGL(k_cell_totals, dim3(256), dim3(256), 0, s, a.stat_pairs, a.stat_acc, a.start_new,
