@@ -173,7 +173,9 @@ dsmc_status dsmc_get_kernel_times(dsmc_handle *h, int *nkernels, char *names, in
 enum {
     DSMC_OPT_COLLIDE_CHUNK = 7, /* cells per collide work-queue entry, 1..64 (0: automatic) */
     DSMC_OPT_GRAPH_STEPS = 8,   /* steps per captured graph, even, 2..64 (default 2) */
-    DSMC_OPT_FUSED = 9          /* 1 (default): gather, collide and sampling in one pass */
+    DSMC_OPT_FUSED = 9,         /* 1 (default): gather, collide and sampling in one pass */
+    DSMC_OPT_MOVE_QUEUE = 10    /* wall-path particles the mover queues per cell, 0..512 (default
+                                 * 512); the rest take the overflow pass */
 };
 typedef struct {
     int graph_steps;            /* chosen steps per captured graph */

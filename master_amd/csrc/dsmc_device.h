@@ -59,12 +59,16 @@ __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
 }
 
 // Load through the scalar data cache (s_load): for tables no kernel writes
-// while it runs (member descriptors), at a wave-uniform address.  The
-// result lands in SGPRs and its wait (lgkmcnt) does not hold back the
-// vector loads in flight.
+// while it runs (member descriptors), at a wave-uniform address (a
+// per-lane address compiles to vector loads).  The result lands in SGPRs
+// and its wait (lgkmcnt) does not hold back the vector loads in flight.
 template <class T>
 __device__ __forceinline__ T ld_const(const T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
     return *(const __attribute__((address_space(4))) T *)p;
+#else
+    return *p;  // host pass of a device function: never executed
+#endif
 }
 
 // (2m+1) 2^-24 with m = x >> 9: exact in float and double, in (0, 1).
@@ -103,7 +107,6 @@ __device__ __forceinline__ uint32_t cell_of(const Member &M, int ix, int iy, int
 // Voxel-type lookup: an LDS tile covering one cell and a one-voxel halo,
 // falling back to the member's voxel array (periodic) outside it.
 struct VoxelView {
-    const uint8_t *types;     // member's types (z-fastest)
     const uint8_t *tile;      // LDS tile
     int tx0, ty0, tz0;        // tile origin (unwrapped voxel coords)
     int TX, TY, TZ;           // tile extents
@@ -115,7 +118,8 @@ struct VoxelView {
             return tile[(a * TY + b) * TZ + c];
         return V_WALL;
     }
-    __device__ __forceinline__ uint32_t at(int ix, int iy, int iz) const {
+    // types: the member's voxel types (z-fastest)
+    __device__ __forceinline__ uint32_t at(const uint8_t *types, int ix, int iy, int iz) const {
         int a = ix - tx0, b = iy - ty0, c = iz - tz0;
         if ((unsigned)a < (unsigned)TX && (unsigned)b < (unsigned)TY && (unsigned)c < (unsigned)TZ)
             return tile[(a * TY + b) * TZ + c];
@@ -250,16 +254,25 @@ __device__ __forceinline__ void wall_collide(const Member &M, const float *frame
 // Coordinates stay unwrapped during the flight (the reference's halo grid)
 // and are wrapped once at the end.  Returns the last wall-hit voxel's linear
 // id or -1.
-__device__ __forceinline__ int32_t move_general(const Member &M, const VoxelView &V,
-                                                const float *frames, uint32_t step,
-                                                uint32_t pidx, int &ix, int &iy, int &iz,
-                                                float &ox, float &oy, float &oz, float &vx,
+//
+// ap points at the kernel's arguments in the kernarg segment (members,
+// types, frames) and mi is the member: the path reads its descriptor and
+// tables itself, behind a compiler barrier, so none of them is hoisted out
+// of the caller's loop to stay in registers for every particle.
+template <class ArgsPtr>
+__device__ __forceinline__ int32_t move_general(ArgsPtr ap, uint32_t mi, const VoxelView &V,
+                                                uint32_t step, uint32_t pidx, int &ix, int &iy,
+                                                int &iz, float &ox, float &oy, float &oz, float &vx,
                                                 float &vy, float &vz, MoveCounters &cnt,
                                                 uint32_t &nhit) {
     // keep the wall and reseed draws' Philox rounds inside this path: the
     // compiler would otherwise hoist their particle-dependent first rounds
     // into every particle's move
     asm volatile("" : "+v"(pidx));
+    asm volatile("" : "+s"(ap));
+    const Member M = ld_const(ap->members + mi);
+    const uint8_t *types = ap->types + M.vox0;
+    const float *frames = ap->frames + 9 * M.vox0;
     // System::accelerate: v[flow_dir] += gravity*dt; adding an exact 0 to the
     // other components leaves them unchanged
     vx = vx + M.acc[0];
@@ -271,7 +284,7 @@ __device__ __forceinline__ int32_t move_general(const Member &M, const VoxelView
     bool reseeded = false;
     for (int depth = 0;; depth++) {
         float tau = dt;
-        if (V.at(ix, iy, iz) != V_EMPTY) { reseeded = true; break; }
+        if (V.at(types, ix, iy, iz) != V_EMPTY) { reseeded = true; break; }
         // Every position of this flight is start + w*tau: the reference's
         // bisection (tau /= 2; r -= v*tau) and its back-off before the face
         // solve (r -= v*tau) return to the flight's start point in exact
@@ -288,7 +301,7 @@ __device__ __forceinline__ int32_t move_general(const Member &M, const VoxelView
             advance1(iz, oz, wz, (T));                                     \
         } while (0)
         DSMC_FLY(tau);
-        uint32_t t = V.at(ix, iy, iz);
+        uint32_t t = V.at(types, ix, iy, iz);
         if (t >= V_WALL) {
             int count = 0;
             bool resumed = false, bad = false;
@@ -297,7 +310,7 @@ __device__ __forceinline__ int32_t move_general(const Member &M, const VoxelView
                 if (t == V_WALL) {
                     tau = tau / 2.0f;
                     DSMC_FLY(tau);
-                    t = V.at(ix, iy, iz);
+                    t = V.at(types, ix, iy, iz);
                 } else {
                     dt = dt - tau;
                     resumed = true;
@@ -324,7 +337,7 @@ __device__ __forceinline__ int32_t move_general(const Member &M, const VoxelView
                     else if (face < 4) settle_outside(face, by, wy, iy, oy);
                     else settle_outside(face, bz, wz, iz, oz);
                 }
-                t = V.at(ix, iy, iz);
+                t = V.at(types, ix, iy, iz);
             }
 #undef DSMC_FLY
             if (bad) { reseeded = true; break; }
@@ -342,7 +355,7 @@ __device__ __forceinline__ int32_t move_general(const Member &M, const VoxelView
     }
     if (reseeded) {
         cnt.reseeds++;
-        if (!reseed(M, V.types, step, pidx, ix, iy, iz, ox, oy, oz)) {
+        if (!reseed(M, types, step, pidx, ix, iy, iz, ox, oy, oz)) {
             // no pore voxel found: keep the particle inside the box
             cnt.errors++;
             ix = ((ix % M.n[0]) + M.n[0]) % M.n[0];
@@ -358,7 +371,7 @@ __device__ __forceinline__ int32_t move_general(const Member &M, const VoxelView
             // more than one box length in one step: cannot happen for a valid dt
             cnt.errors++;
             cnt.reseeds++;
-            if (!reseed(M, V.types, step, pidx, ix, iy, iz, ox, oy, oz)) {
+            if (!reseed(M, types, step, pidx, ix, iy, iz, ox, oy, oz)) {
                 ix = ((ix % M.n[0]) + M.n[0]) % M.n[0];
                 iy = ((iy % M.n[1]) + M.n[1]) % M.n[1];
                 iz = ((iz % M.n[2]) + M.n[2]) % M.n[2];
@@ -369,17 +382,15 @@ __device__ __forceinline__ int32_t move_general(const Member &M, const VoxelView
     return last;
 }
 
-// The common case first: start and landing voxel both empty voxels of the
-// cell's tile.  Its arithmetic is move_general's for that case, operation for
+// The common case: start and landing voxel both empty voxels of the cell's
+// tile.  Its arithmetic is move_general's for that case, operation for
 // operation (accelerate, w = v / h, advance1 per axis, one periodic wrap), so
-// both give bit-identical results; any other particle takes move_general
-// from its original inputs.
-__device__ __forceinline__ int32_t move_particle(const Member &M, const VoxelView &V,
-                                                 const float *frames, uint32_t step,
-                                                 uint32_t pidx, int &ix, int &iy, int &iz,
-                                                 float &ox, float &oy, float &oz, float &vx,
-                                                 float &vy, float &vz, MoveCounters &cnt,
-                                                 uint32_t &nhit) {
+// both give bit-identical results.  Returns false, with the inputs left as
+// they were, for any other particle: that one takes move_general from its
+// original inputs.
+__device__ __forceinline__ bool move_fast(const Member &M, const VoxelView &V, int &ix, int &iy,
+                                          int &iz, float &ox, float &oy, float &oz, float &vx,
+                                          float &vy, float &vz, MoveCounters &cnt) {
     const float ax = vx + M.acc[0], ay = vy + M.acc[1], az = vz + M.acc[2];
     int jx = ix, jy = iy, jz = iz;
     float px = ox, py = oy, pz = oz;
@@ -391,18 +402,17 @@ __device__ __forceinline__ int32_t move_particle(const Member &M, const VoxelVie
         advance1(jz, pz, az * M.inv_h[2], dt);
         fast = V.tile_at(jx, jy, jz) == V_EMPTY;
     }
-    if (fast) {
-        // tile voxels lie within one voxel of the box: one wrap suffices
-        if (jx >= M.n[0]) { jx -= M.n[0]; cnt.periodic[0]++; } else if (jx < 0) { jx += M.n[0]; cnt.periodic[0]--; }
-        if (jy >= M.n[1]) { jy -= M.n[1]; cnt.periodic[1]++; } else if (jy < 0) { jy += M.n[1]; cnt.periodic[1]--; }
-        if (jz >= M.n[2]) { jz -= M.n[2]; cnt.periodic[2]++; } else if (jz < 0) { jz += M.n[2]; cnt.periodic[2]--; }
-        ix = jx; iy = jy; iz = jz;
-        ox = px; oy = py; oz = pz;
-        vx = ax; vy = ay; vz = az;
-        nhit = 0;
-        return -1;
-    }
-    return move_general(M, V, frames, step, pidx, ix, iy, iz, ox, oy, oz, vx, vy, vz, cnt, nhit);
+    if (!fast) return false;
+    // tile voxels lie within one voxel of the box: one wrap suffices, as
+    // selects rather than branches
+    const int wx = (jx >= M.n[0]) - (jx < 0), wy = (jy >= M.n[1]) - (jy < 0), wz = (jz >= M.n[2]) - (jz < 0);
+    jx -= wx * M.n[0]; cnt.periodic[0] += wx;
+    jy -= wy * M.n[1]; cnt.periodic[1] += wy;
+    jz -= wz * M.n[2]; cnt.periodic[2] += wz;
+    ix = jx; iy = jy; iz = jz;
+    ox = px; oy = py; oz = pz;
+    vx = ax; vy = ay; vz = az;
+    return true;
 }
 
 // Wave-level helpers (wave64).

@@ -75,6 +75,7 @@ struct dsmc_handle {
     uint32_t lds_cap = 0;
     bool lds_auto = true;
     uint32_t collide_chunk = 0;     // 0: automatic
+    uint32_t move_queue = MOVE_SLOWQ;  // mover's per-cell wall-path queue in use
     bool fused = true;              // gather and collide in one pass
     int graph_steps = 2;            // steps per captured graph (even)
     // host mirrors of System counters
@@ -260,7 +261,7 @@ dsmc_status upload_geometry(dsmc_handle *h) {
     HIPCHK(h, dalloc(&h->d_bfill, cells));
     HIPCHK(h, dalloc(&h->d_bsums, h->nblocks));
     HIPCHK(h, dalloc(&h->d_step, 1));
-    HIPCHK(h, dalloc(&h->d_mvcount, 2));
+    HIPCHK(h, dalloc(&h->d_mvcount, 3));
     HIPCHK(h, dalloc(&h->d_cnt, CNT_NUM));
     HIPCHK(h, dalloc(&h->d_partial, 2 * TOT_BLOCKS * 4));
     HIPCHK(h, hipMemcpy(h->d_members, members.data(), members.size() * sizeof(Member), hipMemcpyHostToDevice));
@@ -514,7 +515,13 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
         m.counters = h->d_cnt;
         m.rec_nhit = h->record ? h->d_rec_nhit : nullptr;
         m.rec_last = h->record ? h->d_rec_last : nullptr;
+        // the bucket array is free until k_bucket_cells: it holds the
+        // overflow list
+        m.overflow = h->d_bucket;
+        m.n_overflow = h->d_mvcount + 2;
+        m.slowq_cap = h->move_queue;
         m.cell_begin = 0;
+        m.ncells = h->ncells;
         Scope sc(h, "k_move");
         HIPCHK(h, launch_move(m, h->ncells, s));
     } else {
@@ -1157,6 +1164,11 @@ dsmc_status dsmc_set_option(dsmc_handle *h, int option, int64_t value) {
         return DSMC_OK;
     case DSMC_OPT_FUSED:
         h->fused = value != 0;
+        drop_graphs(h);
+        return DSMC_OK;
+    case DSMC_OPT_MOVE_QUEUE:
+        if (value < 0 || value > (int64_t)MOVE_SLOWQ) return DSMC_ERR_INVALID_ARGUMENT;
+        h->move_queue = (uint32_t)value;
         drop_graphs(h);
         return DSMC_OK;
     case DSMC_OPT_GRAPH_STEPS:

@@ -8,6 +8,8 @@
 namespace dsmc {
 
 static constexpr int MAX_TILE = 1000;          // (b+2)^3 voxel tile, b <= 8
+static constexpr uint32_t MOVE_SLOWQ = 512;    // wall-path particles queued per cell in k_move
+static constexpr uint32_t MOVE_OVERFLOW_BLOCKS = 256;
 static constexpr int SCAN_THREADS = 1024;
 static constexpr int SCAN_TILE = 4096;         // cells per scan block
 
@@ -29,7 +31,9 @@ struct MoveArgs {
     const uint32_t *step;
     unsigned long long *counters;
     int32_t *rec_nhit, *rec_last;       // optional recording (per index before the step)
-    uint32_t cell_begin;
+    uint32_t *overflow, *n_overflow;    // particles beyond a cell's queue (n entries), their count
+    uint32_t cell_begin, ncells;
+    uint32_t slowq_cap;                 // queue entries in use, <= MOVE_SLOWQ
 };
 
 struct ClassifyArgs {

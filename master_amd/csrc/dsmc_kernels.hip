@@ -31,7 +31,6 @@ __device__ __forceinline__ void cell_coords(const Member &M, uint32_t lc, int &c
 
 __device__ __forceinline__ void load_tile(const Member &M, const uint8_t *types, uint8_t *tile,
                                           int cx, int cy, int cz, VoxelView &V) {
-    V.types = types;
     V.tile = tile;
     V.tx0 = cx * M.b[0] - 1;
     V.ty0 = cy * M.b[1] - 1;
@@ -66,6 +65,9 @@ __device__ __forceinline__ void load_tile(const Member &M, const uint8_t *types,
 #ifndef DSMC_MOVE_WAVES_PER_EU
 #define DSMC_MOVE_WAVES_PER_EU 1
 #endif
+// REC: also record each particle's wall-hit count and last wall voxel (a
+// test hook; the production instance carries neither pointer)
+template <bool REC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WAVES_PER_EU, 8))) void k_move(MoveArgs a) {
     const uint32_t c = blockIdx.x + a.cell_begin;
     const uint32_t lane = threadIdx.x;
@@ -88,7 +90,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
     }
     const Member M = a.members[mi];
     const uint8_t *types = a.types + M.vox0;
-    const float *frames = a.frames + 9 * M.vox0;
     __shared__ uint8_t tile[MAX_TILE];
     int cx, cy, cz;
     cell_coords(M, c - M.cell0, cx, cy, cz);
@@ -98,8 +99,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
     // tile is visible to every lane without a barrier
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     const uint32_t step = *a.step;
+    // the arguments in the kernarg segment (the kernel's only explicit
+    // argument, at offset 0), for the wall path to read from
+    const auto kargs = (const __attribute__((address_space(4))) MoveArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    // particles the fast path cannot move (a wall, a solid start voxel, a
+    // flight beyond the tile) wait here and are moved together after the
+    // loop, on dense lanes; a full queue spills to the overflow list
+    __shared__ uint32_t slowq[MOVE_SLOWQ];
     MoveCounters cnt = {0, 0, 0, {0, 0, 0}};
-    uint32_t leavers = 0;
+    uint32_t leavers = 0, nslow = 0;
     for (uint32_t base = s0; base < s1; base += WAVE) {
         const uint32_t pn = p + WAVE;
         float nox = 0, noy = 0, noz = 0, nvx = 0, nvy = 0, nvz = 0;
@@ -111,33 +119,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
         }
         const bool valid = p < s1;
         uint32_t newc = c;
+        bool fast = false;
         if (valid) {
             int ix, iy, iz;
             unpack_vox(vp, ix, iy, iz);
-            uint32_t nhit = 0;
-            // draws are keyed by the index inside the member, so a member of an
-            // ensemble steps exactly as it would alone
             const uint32_t v0x = __float_as_uint(vx), v0y = __float_as_uint(vy), v0z = __float_as_uint(vz);
-            int32_t last = move_particle(M, V, frames, step, p - M.pfirst, ix, iy, iz, ox, oy, oz, vx, vy, vz,
-                                         cnt, nhit);
-            // unchanged words are not stored: off the body-force axis a
-            // velocity changes only at a wall, a voxel index only when the
-            // particle crosses a face, so most of their cache lines stay clean
-            a.ox[p] = ox; a.oy[p] = oy; a.oz[p] = oz;
-            if (__float_as_uint(vx) != v0x) a.vx[p] = vx;
-            if (__float_as_uint(vy) != v0y) a.vy[p] = vy;
-            if (__float_as_uint(vz) != v0z) a.vz[p] = vz;
-            const uint32_t vpn = pack_vox(ix, iy, iz);
-            if (vpn != vp) a.vox[p] = vpn;
-            newc = cell_of(M, ix, iy, iz);
-            if (a.rec_nhit) {
-                a.rec_nhit[p] = (int32_t)nhit;
-                a.rec_last[p] = last;
+            fast = move_fast(M, V, ix, iy, iz, ox, oy, oz, vx, vy, vz, cnt);
+            if (fast) {
+                // unchanged words are not stored: off the body-force axis a
+                // velocity changes only at a wall, a voxel index only when
+                // the particle crosses a face, so most of their cache lines
+                // stay clean
+                a.ox[p] = ox; a.oy[p] = oy; a.oz[p] = oz;
+                if (__float_as_uint(vx) != v0x) a.vx[p] = vx;
+                if (__float_as_uint(vy) != v0y) a.vy[p] = vy;
+                if (__float_as_uint(vz) != v0z) a.vz[p] = vz;
+                const uint32_t vpn = pack_vox(ix, iy, iz);
+                if (vpn != vp) a.vox[p] = vpn;
+                newc = cell_of(M, ix, iy, iz);
+                if (REC) {
+                    a.rec_nhit[p] = 0;
+                    a.rec_last[p] = -1;
+                }
             }
         }
-        // movers are listed in the cell's own index range, leavers first:
-        // no global counter, no returning atomic
-        const bool mover = valid && newc != c;
+        const bool slow = valid && !fast;
+        const uint64_t sm = __ballot(slow);
+        if (sm) {
+            if (slow) {
+                const uint32_t q = nslow + mbcnt(sm);
+                if (q < a.slowq_cap) slowq[q] = p;
+                else a.overflow[atomicAdd(a.n_overflow, 1u)] = p;
+            }
+            nslow += (uint32_t)__popcll(sm);
+        }
+        // movers are listed in the cell's own index range: no global
+        // counter, no returning atomic (their order is free: the gather
+        // ranks arrivals by index)
+        const bool mover = fast && newc != c;
         const uint64_t mm = __ballot(mover);
         if (mover) {
             const uint32_t slot = s0 + leavers + mbcnt(mm);
@@ -148,6 +167,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
         leavers += (uint32_t)__popcll(mm);
         p = pn;
         ox = nox; oy = noy; oz = noz; vx = nvx; vy = nvy; vz = nvz; vp = nvp;
+    }
+    // the queued particles, 64 at a time, from their original state
+    const uint32_t nq = nslow < a.slowq_cap ? nslow : a.slowq_cap;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (uint32_t q0 = 0; q0 < nq; q0 += WAVE) {
+        const uint32_t j = q0 + lane;
+        const bool valid = j < nq;
+        uint32_t newc = c, q = 0;
+        if (valid) {
+            q = slowq[j];
+            float qox = a.ox[q], qoy = a.oy[q], qoz = a.oz[q];
+            float qvx = a.vx[q], qvy = a.vy[q], qvz = a.vz[q];
+            int ix, iy, iz;
+            unpack_vox(a.vox[q], ix, iy, iz);
+            uint32_t nhit = 0;
+            // draws are keyed by the index inside the member, so a member of
+            // an ensemble steps exactly as it would alone
+            const int32_t last = move_general(kargs, mi, V, step, q - M.pfirst, ix, iy, iz, qox, qoy, qoz,
+                                              qvx, qvy, qvz, cnt, nhit);
+            a.ox[q] = qox; a.oy[q] = qoy; a.oz[q] = qoz;
+            a.vx[q] = qvx; a.vy[q] = qvy; a.vz[q] = qvz;
+            a.vox[q] = pack_vox(ix, iy, iz);
+            newc = cell_of(M, ix, iy, iz);
+            if (REC) {
+                a.rec_nhit[q] = (int32_t)nhit;
+                a.rec_last[q] = last;
+            }
+        }
+        const bool mover = valid && newc != c;
+        const uint64_t mm = __ballot(mover);
+        if (mover) {
+            const uint32_t slot = s0 + leavers + mbcnt(mm);
+            a.mv_idx[slot] = q;
+            a.mv_cell[slot] = newc;
+            atomicAdd(&a.arrive[newc], 1u);
+        }
+        leavers += (uint32_t)__popcll(mm);
     }
     // wave-uniform already; reduce the per-lane counters
     uint32_t wh = wave_sum(cnt.wall_hits), rs = wave_sum(cnt.reseeds), er = wave_sum(cnt.errors);
@@ -160,6 +216,61 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
         if (p0) atomicAdd(&a.counters[CNT_PER0], (unsigned long long)(long long)p0);
         if (p1) atomicAdd(&a.counters[CNT_PER1], (unsigned long long)(long long)p1);
         if (p2) atomicAdd(&a.counters[CNT_PER2], (unsigned long long)(long long)p2);
+    }
+}
+
+// Particles whose cell's queue was full (k_move), one per lane: the cell
+// from the start array, the move as in k_move's deferred loop, the mover
+// slot from an atomic on the cell's leaver count (k_move has written it).
+__global__ __launch_bounds__(64) void k_move_overflow(MoveArgs a) {
+    const uint32_t n = *a.n_overflow;
+    const uint32_t step = *a.step;
+    const auto kargs = (const __attribute__((address_space(4))) MoveArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    // an empty tile: every voxel lookup goes to the member's voxel array
+    VoxelView V;
+    V.tile = nullptr;
+    V.tx0 = V.ty0 = V.tz0 = 0;
+    V.TX = V.TY = V.TZ = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t q = a.overflow[i];
+        // the cell c with start[c] <= q < start[c + 1]
+        uint32_t lo = 0, hi = a.ncells;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.start[mid] <= q) lo = mid; else hi = mid;
+        }
+        const uint32_t c = lo;
+        const uint32_t mi = a.cell_member[c];
+        const Member M = a.members[mi];
+        V.n0 = M.n[0]; V.n1 = M.n[1]; V.n2 = M.n[2];
+        float qox = a.ox[q], qoy = a.oy[q], qoz = a.oz[q];
+        float qvx = a.vx[q], qvy = a.vy[q], qvz = a.vz[q];
+        int ix, iy, iz;
+        unpack_vox(a.vox[q], ix, iy, iz);
+        MoveCounters cnt = {0, 0, 0, {0, 0, 0}};
+        uint32_t nhit = 0;
+        const int32_t last = move_general(kargs, mi, V, step, q - M.pfirst, ix, iy, iz, qox, qoy, qoz, qvx,
+                                          qvy, qvz, cnt, nhit);
+        a.ox[q] = qox; a.oy[q] = qoy; a.oz[q] = qoz;
+        a.vx[q] = qvx; a.vy[q] = qvy; a.vz[q] = qvz;
+        a.vox[q] = pack_vox(ix, iy, iz);
+        const uint32_t newc = cell_of(M, ix, iy, iz);
+        if (a.rec_nhit) {
+            a.rec_nhit[q] = (int32_t)nhit;
+            a.rec_last[q] = last;
+        }
+        if (newc != c) {
+            const uint32_t slot = a.start[c] + atomicAdd(&a.leave[c], 1u);
+            a.mv_idx[slot] = q;
+            a.mv_cell[slot] = newc;
+            atomicAdd(&a.arrive[newc], 1u);
+        }
+        if (cnt.wall_hits) atomicAdd(&a.counters[CNT_WALL], (unsigned long long)cnt.wall_hits);
+        if (cnt.reseeds) atomicAdd(&a.counters[CNT_RESEED], (unsigned long long)cnt.reseeds);
+        if (cnt.errors) atomicAdd(&a.counters[CNT_ERROR], (unsigned long long)cnt.errors);
+        if (cnt.periodic[0]) atomicAdd(&a.counters[CNT_PER0], (unsigned long long)(long long)cnt.periodic[0]);
+        if (cnt.periodic[1]) atomicAdd(&a.counters[CNT_PER1], (unsigned long long)(long long)cnt.periodic[1]);
+        if (cnt.periodic[2]) atomicAdd(&a.counters[CNT_PER2], (unsigned long long)(long long)cnt.periodic[2]);
     }
 }
 
@@ -1092,6 +1203,7 @@ __global__ void k_begin_step(uint32_t *step, uint32_t *mv_count) {
         *step += 1u;
         mv_count[0] = 0u;
         mv_count[1] = 0u;
+        mv_count[2] = 0u;
     }
 }
 
@@ -1118,7 +1230,12 @@ __global__ __launch_bounds__(256) void k_totals_partial(const float *vx, const f
 
 hipError_t launch_move(const MoveArgs &a, uint32_t ncells, hipStream_t s) {
     if (ncells == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_move, dim3(ncells), dim3(64), 0, s, a);
+    if (a.rec_nhit)
+        hipLaunchKernelGGL(k_move<true>, dim3(ncells), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_move<false>, dim3(ncells), dim3(64), 0, s, a);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_move_overflow, dim3(MOVE_OVERFLOW_BLOCKS), dim3(64), 0, s, a);
     LAUNCH_CHECK();
     return hipSuccess;
 }

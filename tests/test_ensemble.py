@@ -64,9 +64,10 @@ def test_ensemble_member_equals_solo_run(gpu, graph):
 
 @pytest.mark.gpu
 def test_tuning_knobs_leave_results_unchanged(gpu):
-    """Collide chunking, graph length, the fused gather + collide pass and the
-    autotuner's choice change launch shapes only: trajectories stay
-    bit-identical."""
+    """Collide chunking, graph length, the fused gather + collide pass, the
+    mover's wall-path queue (0: every wall-path particle goes through the
+    overflow pass) and the autotuner's choice change launch shapes only:
+    trajectories stay bit-identical."""
     dsmc = gpu
     members = [_member(dsmc, s) for s in SPECS[:2]]
 
@@ -83,11 +84,12 @@ def test_tuning_knobs_leave_results_unchanged(gpu):
     assert rep["ms_per_step"] <= rep["ms_per_step_default"]
     tuned.step(10)
     total = tuned.counters()["steps"]
-    ref = (tuned.state(), tuned.moments()[0], tuned.counters()["collisions"])
+    ref = (tuned.state(), tuned.moments()[0], tuned.counters()["collisions"], tuned.counters()["wall_collisions"])
     tuned.close()
     for knobs in ({}, {dsmc.OPT_COLLIDE_CHUNK: 1, dsmc.OPT_GRAPH_STEPS: 4},
                   {dsmc.OPT_COLLIDE_CHUNK: 16, dsmc.OPT_GRAPH_STEPS: 8},
-                  {dsmc.OPT_FUSED: 0}, {dsmc.OPT_FUSED: 0, dsmc.OPT_USE_GRAPH: 0}):
+                  {dsmc.OPT_FUSED: 0}, {dsmc.OPT_FUSED: 0, dsmc.OPT_USE_GRAPH: 0},
+                  {dsmc.OPT_MOVE_QUEUE: 0}, {dsmc.OPT_MOVE_QUEUE: 3, dsmc.OPT_USE_GRAPH: 0}):
         g = make()
         for k, v in knobs.items():
             g.set_option(k, v)
@@ -96,4 +98,5 @@ def test_tuning_knobs_leave_results_unchanged(gpu):
         assert np.array_equal(g.state(), ref[0]), knobs
         assert np.array_equal(g.moments()[0], ref[1]), knobs
         assert g.counters()["collisions"] == ref[2]
+        assert g.counters()["wall_collisions"] == ref[3]
         g.close()

@@ -49,9 +49,13 @@ def test_seeding_matches_oracle(gpu):
     assert g.counters()["errors"] == 0
 
 
-@pytest.mark.parametrize("wall,mult", [("thermal", 1), ("specular", 1), ("thermal", 10)])
-def test_move_and_sort_match_oracle(gpu, wall, mult):
+# queue 0: every particle off the mover's fast path takes its overflow pass
+@pytest.mark.parametrize("wall,mult,queue", [("thermal", 1, None), ("specular", 1, None), ("thermal", 10, None),
+                                             ("thermal", 10, 0), ("specular", 1, 5)])
+def test_move_and_sort_match_oracle(gpu, wall, mult, queue):
     ini, w, g, o, st0 = _init_pair(gpu, wall=wall, gravity=0.05, dt_tau=0.2 * mult)
+    if queue is not None:
+        g.set_option(K.dsmc.OPT_MOVE_QUEUE, queue)
     g.set_option(K.dsmc.OPT_RECORD, 1)
     g.set_option(K.dsmc.OPT_PHASES, 1)          # move + sort, no collide
     g.step(1)
