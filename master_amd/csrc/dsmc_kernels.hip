@@ -872,29 +872,64 @@ __device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_
     const bool in_lds = n <= K;
 
     // ---- gather (k_gather), velocities to LDS
+    // Round trips: the first stayer batch and the arrival keys go out
+    // together; with at most 64 arrivals (the common case) their ranks come
+    // from the keys in registers and their data is loaded before any store
+    // is issued, so the wait for it does not include a store.
+    float ox[GATHER_BATCH], oy[GATHER_BATCH], oz[GATHER_BATCH], vx[GATHER_BATCH], vy[GATHER_BATCH], vz[GATHER_BATCH];
+    uint32_t vp[GATHER_BATCH];
+#define DSMC_GATHER_LOAD(BASE)                                                         \
+    do {                                                                               \
+        _Pragma("unroll") for (int q = 0; q < GATHER_BATCH; q++) {                     \
+            const uint32_t p = (BASE) + q * WAVE + lane;                               \
+            vp[q] = 0xffffffffu;                                                       \
+            if (p < oe) {                                                              \
+                vp[q] = a.vox[p];                                                      \
+                ox[q] = a.ox[p]; oy[q] = a.oy[p]; oz[q] = a.oz[p];                     \
+                vx[q] = a.vx[p]; vy[q] = a.vy[p]; vz[q] = a.vz[p];                     \
+            }                                                                          \
+        }                                                                              \
+    } while (0)
+    DSMC_GATHER_LOAD(os);
+    const bool few = nb <= WAVE;
+    const uint32_t key0 = lane < nb ? a.bucket[bs + lane] : 0xffffffffu;
     uint32_t n_low = 0;
     const bool keys_lds = nb > WAVE && nb <= MARKS;
-    for (uint32_t i = lane; i < nb; i += WAVE) {
-        const uint32_t k = a.bucket[bs + i];
-        if (keys_lds) keys[i] = k;
-        n_low += k < os ? 1u : 0u;
+    if (few) {
+        n_low = wave_sum(key0 < os ? 1u : 0u);
+    } else {
+        for (uint32_t i = lane; i < nb; i += WAVE) {
+            const uint32_t k = a.bucket[bs + i];
+            if (keys_lds) keys[i] = k;
+            n_low += k < os ? 1u : 0u;
+        }
+        n_low = wave_sum(n_low);
+        if (keys_lds) __syncthreads();
     }
-    n_low = wave_sum(n_low);
-    if (keys_lds) __syncthreads();
+    if (few && nb > 0) {
+        // rank among the arrivals (keys are distinct), then the data
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < nb; j++) {
+            const uint32_t kj = (uint32_t)__shfl((int)key0, (int)j, WAVE);
+            rank += kj < key0 ? 1u : 0u;
+        }
+        if (lane < nb) {
+            const uint32_t q = key0;
+            const float px = a.ox[q], py = a.oy[q], pz = a.oz[q];
+            const uint32_t pv = a.vox[q];
+            const float x = a.vx[q], y = a.vy[q], z = a.vz[q];
+            const uint32_t L = rank < n_low ? rank : n_stay + rank;
+            const uint32_t d = ns + L;
+            a.ox2[d] = px; a.oy2[d] = py; a.oz2[d] = pz;
+            a.vox2[d] = pv;
+            if (in_lds) { lx[L] = x; ly[L] = y; lz[L] = z; }
+            else { a.vx2[d] = x; a.vy2[d] = y; a.vz2[d] = z; }
+            if (a.rec_perm) a.rec_perm[d] = (int32_t)q;
+        }
+    }
     uint32_t run = 0;
     for (uint32_t base = os; base < oe; base += GATHER_BATCH * WAVE) {
-        float ox[GATHER_BATCH], oy[GATHER_BATCH], oz[GATHER_BATCH], vx[GATHER_BATCH], vy[GATHER_BATCH], vz[GATHER_BATCH];
-        uint32_t vp[GATHER_BATCH];
-#pragma unroll
-        for (int q = 0; q < GATHER_BATCH; q++) {
-            const uint32_t p = base + q * WAVE + lane;
-            vp[q] = 0xffffffffu;
-            if (p < oe) {
-                vp[q] = a.vox[p];
-                ox[q] = a.ox[p]; oy[q] = a.oy[p]; oz[q] = a.oz[p];
-                vx[q] = a.vx[p]; vy[q] = a.vy[p]; vz[q] = a.vz[p];
-            }
-        }
+        if (base != os) DSMC_GATHER_LOAD(base);
 #pragma unroll
         for (int q = 0; q < GATHER_BATCH; q++) {
             const uint32_t p = base + q * WAVE + lane;
@@ -917,17 +952,14 @@ __device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_
             run += (uint32_t)__popcll(sm);
         }
     }
-    for (uint32_t base = 0; base < nb; base += WAVE) {
+#undef DSMC_GATHER_LOAD
+    // more than 64 arrivals: ranks from the keys in LDS, or in the bucket
+    for (uint32_t base = 0; !few && base < nb; base += WAVE) {
         const uint32_t i = base + lane;
         const bool valid = i < nb;
         const uint32_t key = valid ? a.bucket[bs + i] : 0xffffffffu;
         uint32_t rank = 0;
-        if (nb <= WAVE) {
-            for (uint32_t j = 0; j < nb; j++) {
-                const uint32_t kj = (uint32_t)__shfl((int)key, (int)j, WAVE);
-                rank += kj < key ? 1u : 0u;
-            }
-        } else if (keys_lds) {
+        if (keys_lds) {
             for (uint32_t j = 0; j < nb; j++) rank += keys[j] < key ? 1u : 0u;
         } else {
             for (uint32_t j = 0; j < nb; j++) rank += a.bucket[bs + j] < key ? 1u : 0u;
