@@ -99,9 +99,20 @@ __device__ __forceinline__ int wrapi(int i, int n) {
 // the next integer, so the float product truncates to the exact quotient.
 __device__ __forceinline__ int div_b(int i, float inv_b) { return (int)((float)i * inv_b); }
 
+// (cells per axis <= 1024: 24-bit products)
 __device__ __forceinline__ uint32_t cell_of(const Member &M, int ix, int iy, int iz) {
     int cx = div_b(ix, M.inv_b[0]), cy = div_b(iy, M.inv_b[1]), cz = div_b(iz, M.inv_b[2]);
-    return M.cell0 + (uint32_t)((cx * M.cpa[1] + cy) * M.cpa[2] + cz);
+    return M.cell0 + __umul24(__umul24((uint32_t)cx, (uint32_t)M.cpa[1]) + (uint32_t)cy, (uint32_t)M.cpa[2]) +
+           (uint32_t)cz;
+}
+
+// Element of an array of 4-byte words at byte offset off = 4 i (arrays of at
+// most 2^30 elements, so off < 2^32): addressed as the SGPR base plus a
+// 32-bit VGPR offset, no 64-bit address arithmetic per access.
+template <class T>
+__device__ __forceinline__ T &at_off(T *base, uint32_t off) {
+    static_assert(sizeof(T) == 4, "4-byte elements");
+    return *(T *)((char *)base + off);
 }
 
 // Voxel-type lookup: an LDS tile covering one cell and a one-voxel halo,
@@ -112,11 +123,13 @@ struct VoxelView {
     int TX, TY, TZ;           // tile extents
     int n0, n1, n2;
     // the tile only; V_WALL outside it
+    // (no branch: outside the tile it reads voxel 0 and returns V_WALL)
     __device__ __forceinline__ uint32_t tile_at(int ix, int iy, int iz) const {
-        int a = ix - tx0, b = iy - ty0, c = iz - tz0;
-        if ((unsigned)a < (unsigned)TX && (unsigned)b < (unsigned)TY && (unsigned)c < (unsigned)TZ)
-            return tile[(a * TY + b) * TZ + c];
-        return V_WALL;
+        const uint32_t a = (uint32_t)(ix - tx0), b = (uint32_t)(iy - ty0), c = (uint32_t)(iz - tz0);
+        const bool in = (a < (uint32_t)TX) & (b < (uint32_t)TY) & (c < (uint32_t)TZ);
+        const uint32_t i = __umul24(__umul24(in ? a : 0u, (uint32_t)TY) + (in ? b : 0u), (uint32_t)TZ) + (in ? c : 0u);
+        const uint32_t t = tile[i];
+        return in ? t : (uint32_t)V_WALL;
     }
     // types: the member's voxel types (z-fastest)
     __device__ __forceinline__ uint32_t at(const uint8_t *types, int ix, int iy, int iz) const {
@@ -385,34 +398,29 @@ __device__ __forceinline__ int32_t move_general(ArgsPtr ap, uint32_t mi, const V
 // The common case: start and landing voxel both empty voxels of the cell's
 // tile.  Its arithmetic is move_general's for that case, operation for
 // operation (accelerate, w = v / h, advance1 per axis, one periodic wrap), so
-// both give bit-identical results.  Returns false, with the inputs left as
-// they were, for any other particle: that one takes move_general from its
-// original inputs.
+// both give bit-identical results.  Computed without branches for every
+// lane; returns false for any other particle, whose outputs are then
+// meaningless (it takes move_general from its original inputs).
 __device__ __forceinline__ bool move_fast(const Member &M, const VoxelView &V, int &ix, int &iy,
                                           int &iz, float &ox, float &oy, float &oz, float &vx,
                                           float &vy, float &vz, MoveCounters &cnt) {
     const float ax = vx + M.acc[0], ay = vy + M.acc[1], az = vz + M.acc[2];
     int jx = ix, jy = iy, jz = iz;
-    float px = ox, py = oy, pz = oz;
-    bool fast = V.tile_at(ix, iy, iz) == V_EMPTY;
-    if (fast) {
-        const float dt = M.dt;
-        advance1(jx, px, ax * M.inv_h[0], dt);
-        advance1(jy, py, ay * M.inv_h[1], dt);
-        advance1(jz, pz, az * M.inv_h[2], dt);
-        fast = V.tile_at(jx, jy, jz) == V_EMPTY;
-    }
-    if (!fast) return false;
-    // tile voxels lie within one voxel of the box: one wrap suffices, as
-    // selects rather than branches
+    const float dt = M.dt;
+    advance1(jx, ox, ax * M.inv_h[0], dt);
+    advance1(jy, oy, ay * M.inv_h[1], dt);
+    advance1(jz, oz, az * M.inv_h[2], dt);
+    const bool fast = (V.tile_at(ix, iy, iz) == V_EMPTY) & (V.tile_at(jx, jy, jz) == V_EMPTY);
+    // tile voxels lie within one voxel of the box: one wrap suffices
     const int wx = (jx >= M.n[0]) - (jx < 0), wy = (jy >= M.n[1]) - (jy < 0), wz = (jz >= M.n[2]) - (jz < 0);
-    jx -= wx * M.n[0]; cnt.periodic[0] += wx;
-    jy -= wy * M.n[1]; cnt.periodic[1] += wy;
-    jz -= wz * M.n[2]; cnt.periodic[2] += wz;
-    ix = jx; iy = jy; iz = jz;
-    ox = px; oy = py; oz = pz;
+    cnt.periodic[0] += fast ? wx : 0;
+    cnt.periodic[1] += fast ? wy : 0;
+    cnt.periodic[2] += fast ? wz : 0;
+    ix = wx > 0 ? jx - M.n[0] : (wx < 0 ? jx + M.n[0] : jx);
+    iy = wy > 0 ? jy - M.n[1] : (wy < 0 ? jy + M.n[1] : jy);
+    iz = wz > 0 ? jz - M.n[2] : (wz < 0 ? jz + M.n[2] : jz);
     vx = ax; vy = ay; vz = az;
-    return true;
+    return fast;
 }
 
 // Wave-level helpers (wave64).

@@ -303,7 +303,8 @@ void drop_graphs(dsmc_handle *h) {
 dsmc_status alloc_particles(dsmc_handle *h, int64_t n) {
     free_particles(h);
     drop_graphs(h);
-    if (n > (int64_t)0xFFFFFFF0ll) return fail(h, DSMC_ERR_CAPACITY, "more than 2^32 particles");
+    // kernels address particle arrays with 32-bit byte offsets
+    if (n > (int64_t)(1ll << 30)) return fail(h, DSMC_ERR_CAPACITY, "more than 2^30 particles");
     h->N = n;
     for (int b = 0; b < 2; b++) {
         HIPCHK(h, dalloc(&h->ox[b], n)); HIPCHK(h, dalloc(&h->oy[b], n)); HIPCHK(h, dalloc(&h->oz[b], n));

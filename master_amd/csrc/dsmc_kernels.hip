@@ -80,13 +80,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
     // software pipeline: the first chunk's loads go out before the member
     // descriptor and the voxel tile are fetched, the next chunk's while this
     // chunk is moved
+    // lanes past the cell's end load its last particle (no branch) and
+    // store nothing
     uint32_t p = s0 + lane;
-    float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
-    uint32_t vp = 0;
-    if (p < s1) {
-        ox = a.ox[p]; oy = a.oy[p]; oz = a.oz[p];
-        vx = a.vx[p]; vy = a.vy[p]; vz = a.vz[p];
-        vp = a.vox[p];
+    float ox, oy, oz, vx, vy, vz;
+    uint32_t vp;
+    {
+        const uint32_t o = (p < s1 ? p : s1 - 1) << 2;
+        ox = at_off(a.ox, o); oy = at_off(a.oy, o); oz = at_off(a.oz, o);
+        vx = at_off(a.vx, o); vy = at_off(a.vy, o); vz = at_off(a.vz, o);
+        vp = at_off(a.vox, o);
     }
     const Member M = a.members[mi];
     const uint8_t *types = a.types + M.vox0;
@@ -110,36 +113,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
     uint32_t leavers = 0, nslow = 0;
     for (uint32_t base = s0; base < s1; base += WAVE) {
         const uint32_t pn = p + WAVE;
-        float nox = 0, noy = 0, noz = 0, nvx = 0, nvy = 0, nvz = 0;
-        uint32_t nvp = 0;
-        if (pn < s1) {
-            nox = a.ox[pn]; noy = a.oy[pn]; noz = a.oz[pn];
-            nvx = a.vx[pn]; nvy = a.vy[pn]; nvz = a.vz[pn];
-            nvp = a.vox[pn];
+        float nox, noy, noz, nvx, nvy, nvz;
+        uint32_t nvp;
+        {
+            const uint32_t o = (pn < s1 ? pn : s1 - 1) << 2;
+            nox = at_off(a.ox, o); noy = at_off(a.oy, o); noz = at_off(a.oz, o);
+            nvx = at_off(a.vx, o); nvy = at_off(a.vy, o); nvz = at_off(a.vz, o);
+            nvp = at_off(a.vox, o);
         }
         const bool valid = p < s1;
+        const uint32_t po = p << 2;
         uint32_t newc = c;
-        bool fast = false;
-        if (valid) {
-            int ix, iy, iz;
-            unpack_vox(vp, ix, iy, iz);
-            const uint32_t v0x = __float_as_uint(vx), v0y = __float_as_uint(vy), v0z = __float_as_uint(vz);
-            fast = move_fast(M, V, ix, iy, iz, ox, oy, oz, vx, vy, vz, cnt);
+        // every lane computes (past the end on a copy of the last particle)
+        int ix, iy, iz;
+        unpack_vox(vp, ix, iy, iz);
+        const uint32_t v0x = __float_as_uint(vx), v0y = __float_as_uint(vy), v0z = __float_as_uint(vz);
+        MoveCounters step_cnt = {0, 0, 0, {0, 0, 0}};
+        const bool fast = move_fast(M, V, ix, iy, iz, ox, oy, oz, vx, vy, vz, step_cnt) & valid;
+        {
             if (fast) {
+                cnt.periodic[0] += step_cnt.periodic[0];
+                cnt.periodic[1] += step_cnt.periodic[1];
+                cnt.periodic[2] += step_cnt.periodic[2];
                 // unchanged words are not stored: off the body-force axis a
                 // velocity changes only at a wall, a voxel index only when
                 // the particle crosses a face, so most of their cache lines
                 // stay clean
-                a.ox[p] = ox; a.oy[p] = oy; a.oz[p] = oz;
-                if (__float_as_uint(vx) != v0x) a.vx[p] = vx;
-                if (__float_as_uint(vy) != v0y) a.vy[p] = vy;
-                if (__float_as_uint(vz) != v0z) a.vz[p] = vz;
+                at_off(a.ox, po) = ox; at_off(a.oy, po) = oy; at_off(a.oz, po) = oz;
+                if (__float_as_uint(vx) != v0x) at_off(a.vx, po) = vx;
+                if (__float_as_uint(vy) != v0y) at_off(a.vy, po) = vy;
+                if (__float_as_uint(vz) != v0z) at_off(a.vz, po) = vz;
                 const uint32_t vpn = pack_vox(ix, iy, iz);
-                if (vpn != vp) a.vox[p] = vpn;
-                newc = cell_of(M, ix, iy, iz);
+                if (vpn != vp) at_off(a.vox, po) = vpn;
+                newc = vpn == vp ? c : cell_of(M, ix, iy, iz);
                 if (REC) {
-                    a.rec_nhit[p] = 0;
-                    a.rec_last[p] = -1;
+                    at_off(a.rec_nhit, po) = 0;
+                    at_off(a.rec_last, po) = -1;
                 }
             }
         }
@@ -159,9 +168,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
         const bool mover = fast && newc != c;
         const uint64_t mm = __ballot(mover);
         if (mover) {
-            const uint32_t slot = s0 + leavers + mbcnt(mm);
-            a.mv_idx[slot] = p;
-            a.mv_cell[slot] = newc;
+            const uint32_t so = (s0 + leavers + mbcnt(mm)) << 2;
+            at_off(a.mv_idx, so) = p;
+            at_off(a.mv_cell, so) = newc;
             atomicAdd(&a.arrive[newc], 1u);
         }
         leavers += (uint32_t)__popcll(mm);
@@ -177,22 +186,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
         uint32_t newc = c, q = 0;
         if (valid) {
             q = slowq[j];
-            float qox = a.ox[q], qoy = a.oy[q], qoz = a.oz[q];
-            float qvx = a.vx[q], qvy = a.vy[q], qvz = a.vz[q];
+            const uint32_t qo = q << 2;
+            float qox = at_off(a.ox, qo), qoy = at_off(a.oy, qo), qoz = at_off(a.oz, qo);
+            float qvx = at_off(a.vx, qo), qvy = at_off(a.vy, qo), qvz = at_off(a.vz, qo);
             int ix, iy, iz;
-            unpack_vox(a.vox[q], ix, iy, iz);
+            unpack_vox(at_off(a.vox, qo), ix, iy, iz);
             uint32_t nhit = 0;
             // draws are keyed by the index inside the member, so a member of
             // an ensemble steps exactly as it would alone
             const int32_t last = move_general(kargs, mi, V, step, q - M.pfirst, ix, iy, iz, qox, qoy, qoz,
                                               qvx, qvy, qvz, cnt, nhit);
-            a.ox[q] = qox; a.oy[q] = qoy; a.oz[q] = qoz;
-            a.vx[q] = qvx; a.vy[q] = qvy; a.vz[q] = qvz;
-            a.vox[q] = pack_vox(ix, iy, iz);
+            at_off(a.ox, qo) = qox; at_off(a.oy, qo) = qoy; at_off(a.oz, qo) = qoz;
+            at_off(a.vx, qo) = qvx; at_off(a.vy, qo) = qvy; at_off(a.vz, qo) = qvz;
+            at_off(a.vox, qo) = pack_vox(ix, iy, iz);
             newc = cell_of(M, ix, iy, iz);
             if (REC) {
-                a.rec_nhit[q] = (int32_t)nhit;
-                a.rec_last[q] = last;
+                at_off(a.rec_nhit, qo) = (int32_t)nhit;
+                at_off(a.rec_last, qo) = last;
             }
         }
         const bool mover = valid && newc != c;
