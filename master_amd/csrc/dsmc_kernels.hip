@@ -423,6 +423,11 @@ __global__ __launch_bounds__(256) void k_bucket_cells(BucketArgs a) {
 // so the kernel runs at full occupancy.
 static constexpr int GATHER_KEYS = 512;
 static constexpr int GATHER_BATCH = 8;      // chunks of 64 particles per round trip
+// The fused pass's batch (chunks of 64 old particles per round trip) is a
+// template parameter: 8 (124 VGPRs, four waves per SIMD) or 14 (166 VGPRs,
+// three), the larger where a cell's velocities in LDS hold occupancy at
+// three waves per SIMD anyway (see launch_gather_collide).
+static constexpr int GC_BATCH_SMALL = 8, GC_BATCH_LARGE = 14;
 
 __global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
     const uint32_t c = blockIdx.x + a.cell_begin;
@@ -864,6 +869,7 @@ struct GcMeta {
     double coeff, vrm, rest;
 };
 
+template <int GC_BATCH>
 __device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_t c, const GcMeta &m,
                                                     const Member &M, float *lx, float *ly, float *lz,
                                                     uint32_t *mark, uint32_t K) {
@@ -886,11 +892,11 @@ __device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_
     // together; with at most 64 arrivals (the common case) their ranks come
     // from the keys in registers and their data is loaded before any store
     // is issued, so the wait for it does not include a store.
-    float ox[GATHER_BATCH], oy[GATHER_BATCH], oz[GATHER_BATCH], vx[GATHER_BATCH], vy[GATHER_BATCH], vz[GATHER_BATCH];
-    uint32_t vp[GATHER_BATCH];
+    float ox[GC_BATCH], oy[GC_BATCH], oz[GC_BATCH], vx[GC_BATCH], vy[GC_BATCH], vz[GC_BATCH];
+    uint32_t vp[GC_BATCH];
 #define DSMC_GATHER_LOAD(BASE)                                                         \
     do {                                                                               \
-        _Pragma("unroll") for (int q = 0; q < GATHER_BATCH; q++) {                     \
+        _Pragma("unroll") for (int q = 0; q < GC_BATCH; q++) {                     \
             const uint32_t p = (BASE) + q * WAVE + lane;                               \
             vp[q] = 0xffffffffu;                                                       \
             if (p < oe) {                                                              \
@@ -938,10 +944,10 @@ __device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_
         }
     }
     uint32_t run = 0;
-    for (uint32_t base = os; base < oe; base += GATHER_BATCH * WAVE) {
+    for (uint32_t base = os; base < oe; base += GC_BATCH * WAVE) {
         if (base != os) DSMC_GATHER_LOAD(base);
 #pragma unroll
-        for (int q = 0; q < GATHER_BATCH; q++) {
+        for (int q = 0; q < GC_BATCH; q++) {
             const uint32_t p = base + q * WAVE + lane;
             bool stay = false;
             if (p < oe) {
@@ -1077,6 +1083,7 @@ __device__ __forceinline__ GcBatch load_gc_batch(const GatherArgs &a, uint32_t c
     return b;
 }
 
+template <int GC_BATCH>
 __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
     const uint32_t K = a.lds_cap;
     float *lx = dyn_lds, *ly = dyn_lds + K, *lz = dyn_lds + 2 * K;
@@ -1106,7 +1113,7 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
             M = a.members[mi];
             member = mi;
         }
-        gather_collide_cell(a, c, m, M, lx, ly, lz, mark, K);
+        gather_collide_cell<GC_BATCH>(a, c, m, M, lx, ly, lz, mark, K);
         if (++j == a.chunk) {
             cur = nxt;
             q0 = q1;
@@ -1328,7 +1335,11 @@ hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t 
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_collide, 64, lds);
+    // LDS for at most three waves per SIMD (12 per CU): the large batch
+    // costs no occupancy
+    const bool large = lds * 16 > (size_t)160 * 1024;
+    e = large ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_collide<GC_BATCH_LARGE>, 64, lds)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_collide<GC_BATCH_SMALL>, 64, lds);
     if (e != hipSuccess) return e;
     uint32_t grid = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
     GatherArgs b = a;
@@ -1337,7 +1348,10 @@ hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t 
     if (!chunk_opt && b.chunk > 8u) b.chunk = 8u;
     const uint32_t nchunks = (ncells + b.chunk - 1) / b.chunk;
     if (grid > nchunks) grid = nchunks;
-    hipLaunchKernelGGL(k_gather_collide, dim3(grid), dim3(64), lds, s, b);
+    if (large)
+        hipLaunchKernelGGL(k_gather_collide<GC_BATCH_LARGE>, dim3(grid), dim3(64), lds, s, b);
+    else
+        hipLaunchKernelGGL(k_gather_collide<GC_BATCH_SMALL>, dim3(grid), dim3(64), lds, s, b);
     LAUNCH_CHECK();
     if (a.do_collide) {
         hipLaunchKernelGGL(k_cell_totals, dim3(256), dim3(256), 0, s, a.stat_pairs, a.stat_acc, a.start_new,

@@ -66,7 +66,8 @@ def test_ensemble_member_equals_solo_run(gpu, graph):
 def test_tuning_knobs_leave_results_unchanged(gpu):
     """Collide chunking, graph length, the fused gather + collide pass, the
     mover's wall-path queue (0: every wall-path particle goes through the
-    overflow pass) and the autotuner's choice change launch shapes only:
+    overflow pass), the LDS capacity (1024: the fused pass's large-batch
+    variant) and the autotuner's choice change launch shapes only:
     trajectories stay bit-identical."""
     dsmc = gpu
     members = [_member(dsmc, s) for s in SPECS[:2]]
@@ -89,7 +90,8 @@ def test_tuning_knobs_leave_results_unchanged(gpu):
     for knobs in ({}, {dsmc.OPT_COLLIDE_CHUNK: 1, dsmc.OPT_GRAPH_STEPS: 4},
                   {dsmc.OPT_COLLIDE_CHUNK: 16, dsmc.OPT_GRAPH_STEPS: 8},
                   {dsmc.OPT_FUSED: 0}, {dsmc.OPT_FUSED: 0, dsmc.OPT_USE_GRAPH: 0},
-                  {dsmc.OPT_MOVE_QUEUE: 0}, {dsmc.OPT_MOVE_QUEUE: 3, dsmc.OPT_USE_GRAPH: 0}):
+                  {dsmc.OPT_MOVE_QUEUE: 0}, {dsmc.OPT_MOVE_QUEUE: 3, dsmc.OPT_USE_GRAPH: 0},
+                  {dsmc.OPT_LDS_CAPACITY: 1024}):
         g = make()
         for k, v in knobs.items():
             g.set_option(k, v)
