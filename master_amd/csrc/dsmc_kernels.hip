@@ -424,10 +424,11 @@ __global__ __launch_bounds__(256) void k_bucket_cells(BucketArgs a) {
 static constexpr int GATHER_KEYS = 512;
 static constexpr int GATHER_BATCH = 8;      // chunks of 64 particles per round trip
 // The fused pass's batch (chunks of 64 old particles per round trip) is a
-// template parameter: 8 (124 VGPRs, four waves per SIMD) or 14 (166 VGPRs,
+// template parameter: 5 (120 VGPRs, four waves per SIMD) or 11 (162 VGPRs,
 // three), the larger where a cell's velocities in LDS hold occupancy at
-// three waves per SIMD anyway (see launch_gather_collide).
-static constexpr int GC_BATCH_SMALL = 8, GC_BATCH_LARGE = 14;
+// three waves per SIMD anyway (see launch_gather_collide).  The next cell's
+// batch is in registers during a cell's write-back.
+static constexpr int GC_BATCH_SMALL = 5, GC_BATCH_LARGE = 11;
 
 __global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
     const uint32_t c = blockIdx.x + a.cell_begin;
@@ -869,10 +870,44 @@ struct GcMeta {
     double coeff, vrm, rest;
 };
 
+// A cell's first batch of old particles and its arrival keys, loaded one
+// cell ahead (k_gather_collide issues them before the previous cell's
+// write-back stores).
 template <int GC_BATCH>
-__device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_t c, const GcMeta &m,
-                                                    const Member &M, float *lx, float *ly, float *lz,
-                                                    uint32_t *mark, uint32_t K) {
+struct GcAhead {
+    float ox[GC_BATCH], oy[GC_BATCH], oz[GC_BATCH], vx[GC_BATCH], vy[GC_BATCH], vz[GC_BATCH];
+    uint32_t vp[GC_BATCH];
+    uint32_t key0;
+};
+
+template <int GC_BATCH>
+__device__ __forceinline__ void gc_load_batch(const GatherArgs &a, uint32_t base, uint32_t oe, GcAhead<GC_BATCH> &P) {
+    const uint32_t lane = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < GC_BATCH; q++) {
+        const uint32_t p = base + q * WAVE + lane;
+        P.vp[q] = 0xffffffffu;
+        if (p < oe) {
+            P.vp[q] = a.vox[p];
+            P.ox[q] = a.ox[p]; P.oy[q] = a.oy[p]; P.oz[q] = a.oz[p];
+            P.vx[q] = a.vx[p]; P.vy[q] = a.vy[p]; P.vz[q] = a.vz[p];
+        }
+    }
+}
+
+template <int GC_BATCH>
+__device__ __forceinline__ void gc_load_ahead(const GatherArgs &a, const GcMeta &m, GcAhead<GC_BATCH> &P) {
+    gc_load_batch<GC_BATCH>(a, m.os, m.oe, P);
+    const uint32_t nb = m.be - m.bs;
+    P.key0 = threadIdx.x < nb ? a.bucket[m.bs + threadIdx.x] : 0xffffffffu;
+}
+
+// Gather (the first batch and the keys already loaded in P) and collide one
+// cell; returns whether its write-back (gc_write_back) is due.
+template <int GC_BATCH>
+__device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t c, const GcMeta &m,
+                                                  const Member &M, GcAhead<GC_BATCH> &P, float *lx, float *ly,
+                                                  float *lz, uint32_t *mark, uint32_t K) {
     const uint32_t lane = threadIdx.x;
     // the mark table is free until the NTC loop: it holds the arrival keys
     uint32_t *keys = mark;
@@ -880,7 +915,7 @@ __device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_
     const uint32_t n = ne - ns;
     const double coeff = m.coeff;
     const bool active = coeff > 0.0;
-    if (n == 0 && !(active && a.do_collide)) return;
+    if (n == 0 && !(active && a.do_collide)) return false;
     const uint32_t os = m.os, oe = m.oe;
     const uint32_t bs = m.bs, be = m.be;
     const uint32_t nb = be - bs;
@@ -888,27 +923,12 @@ __device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_
     const bool in_lds = n <= K;
 
     // ---- gather (k_gather), velocities to LDS
-    // Round trips: the first stayer batch and the arrival keys go out
-    // together; with at most 64 arrivals (the common case) their ranks come
-    // from the keys in registers and their data is loaded before any store
-    // is issued, so the wait for it does not include a store.
-    float ox[GC_BATCH], oy[GC_BATCH], oz[GC_BATCH], vx[GC_BATCH], vy[GC_BATCH], vz[GC_BATCH];
-    uint32_t vp[GC_BATCH];
-#define DSMC_GATHER_LOAD(BASE)                                                         \
-    do {                                                                               \
-        _Pragma("unroll") for (int q = 0; q < GC_BATCH; q++) {                     \
-            const uint32_t p = (BASE) + q * WAVE + lane;                               \
-            vp[q] = 0xffffffffu;                                                       \
-            if (p < oe) {                                                              \
-                vp[q] = a.vox[p];                                                      \
-                ox[q] = a.ox[p]; oy[q] = a.oy[p]; oz[q] = a.oz[p];                     \
-                vx[q] = a.vx[p]; vy[q] = a.vy[p]; vz[q] = a.vz[p];                     \
-            }                                                                          \
-        }                                                                              \
-    } while (0)
-    DSMC_GATHER_LOAD(os);
+    // Round trips: the first stayer batch and the arrival keys were loaded
+    // together, ahead; with at most 64 arrivals (the common case) their ranks
+    // come from the keys in registers and their data is loaded before any
+    // store of this cell is issued.
     const bool few = nb <= WAVE;
-    const uint32_t key0 = lane < nb ? a.bucket[bs + lane] : 0xffffffffu;
+    const uint32_t key0 = P.key0;
     uint32_t n_low = 0;
     const bool keys_lds = nb > WAVE && nb <= MARKS;
     if (few) {
@@ -945,30 +965,29 @@ __device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_
     }
     uint32_t run = 0;
     for (uint32_t base = os; base < oe; base += GC_BATCH * WAVE) {
-        if (base != os) DSMC_GATHER_LOAD(base);
+        if (base != os) gc_load_batch<GC_BATCH>(a, base, oe, P);
 #pragma unroll
         for (int q = 0; q < GC_BATCH; q++) {
             const uint32_t p = base + q * WAVE + lane;
             bool stay = false;
             if (p < oe) {
                 int ix, iy, iz;
-                unpack_vox(vp[q], ix, iy, iz);
+                unpack_vox(P.vp[q], ix, iy, iz);
                 stay = cell_of(M, ix, iy, iz) == c;
             }
             const uint64_t sm = __ballot(stay);
             if (stay) {
                 const uint32_t L = n_low + run + mbcnt(sm);
                 const uint32_t d = ns + L;
-                a.ox2[d] = ox[q]; a.oy2[d] = oy[q]; a.oz2[d] = oz[q];
-                a.vox2[d] = vp[q];
-                if (in_lds) { lx[L] = vx[q]; ly[L] = vy[q]; lz[L] = vz[q]; }
-                else { a.vx2[d] = vx[q]; a.vy2[d] = vy[q]; a.vz2[d] = vz[q]; }
+                a.ox2[d] = P.ox[q]; a.oy2[d] = P.oy[q]; a.oz2[d] = P.oz[q];
+                a.vox2[d] = P.vp[q];
+                if (in_lds) { lx[L] = P.vx[q]; ly[L] = P.vy[q]; lz[L] = P.vz[q]; }
+                else { a.vx2[d] = P.vx[q]; a.vy2[d] = P.vy[q]; a.vz2[d] = P.vz[q]; }
                 if (a.rec_perm) a.rec_perm[d] = (int32_t)p;
             }
             run += (uint32_t)__popcll(sm);
         }
     }
-#undef DSMC_GATHER_LOAD
     // more than 64 arrivals: ranks from the keys in LDS, or in the bucket
     for (uint32_t base = 0; !few && base < nb; base += WAVE) {
         const uint32_t i = base + lane;
@@ -1028,8 +1047,17 @@ __device__ __forceinline__ void gather_collide_cell(const GatherArgs &a, uint32_
             a.stat_acc[c] = (int32_t)acc;
         }
     }
-    if (n == 0) return;
-    // ---- write-back and per-cell FP64 moments (paths apart, as in k_collide)
+    return n > 0;
+}
+
+// Velocities of a gathered and collided cell back to the new buffer, and
+// its FP64 moments (paths apart, as in k_collide).
+__device__ __forceinline__ void gc_write_back(const GatherArgs &a, uint32_t c, const GcMeta &m, const float *lx,
+                                              const float *ly, const float *lz, uint32_t K) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t ns = m.ns, n = m.ne - m.ns;
+    const bool in_lds = n <= K;
+    const bool active = m.coeff > 0.0;
     double mm[7] = {0, 0, 0, 0, 0, 0, 0};
     if (in_lds) {
         for (uint32_t i = lane; i < n; i += WAVE) {
@@ -1094,26 +1122,37 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
     GcBatch nxt = load_gc_batch(a, q1);
     uint32_t j = 0, member = 0xffffffffu;
     Member M;
-    for (;;) {
-        const uint64_t c64 = (uint64_t)q0 * a.chunk + j;
-        if (c64 >= a.ncells) break;
+    GcMeta m;
+    uint32_t mi = 0;
+#define DSMC_GC_META(J)                                                  \
+    do {                                                                 \
+        m.ns = __builtin_amdgcn_readlane(cur.ns, (J));                   \
+        m.ne = __builtin_amdgcn_readlane(cur.ne, (J));                   \
+        m.os = __builtin_amdgcn_readlane(cur.os, (J));                   \
+        m.oe = __builtin_amdgcn_readlane(cur.oe, (J));                   \
+        m.bs = __builtin_amdgcn_readlane(cur.bs, (J));                   \
+        m.be = __builtin_amdgcn_readlane(cur.be, (J));                   \
+        m.coeff = readlane_d(cur.coeff, (J));                            \
+        m.vrm = readlane_d(cur.vrm, (J));                                \
+        m.rest = readlane_d(cur.rest, (J));                              \
+        mi = __builtin_amdgcn_readlane(cur.member, (J));                 \
+    } while (0)
+    uint64_t c64 = (uint64_t)q0 * a.chunk;
+    GcAhead<GC_BATCH> P;
+    if (c64 < a.ncells) {
+        DSMC_GC_META(0);
+        gc_load_ahead<GC_BATCH>(a, m, P);
+    }
+    while (c64 < a.ncells) {
         const uint32_t c = (uint32_t)c64;
-        GcMeta m;
-        m.ns = __builtin_amdgcn_readlane(cur.ns, j);
-        m.ne = __builtin_amdgcn_readlane(cur.ne, j);
-        m.os = __builtin_amdgcn_readlane(cur.os, j);
-        m.oe = __builtin_amdgcn_readlane(cur.oe, j);
-        m.bs = __builtin_amdgcn_readlane(cur.bs, j);
-        m.be = __builtin_amdgcn_readlane(cur.be, j);
-        m.coeff = readlane_d(cur.coeff, j);
-        m.vrm = readlane_d(cur.vrm, j);
-        m.rest = readlane_d(cur.rest, j);
-        const uint32_t mi = __builtin_amdgcn_readlane(cur.member, j);
         if (mi != member) {            // members change only at member boundaries
             M = a.members[mi];
             member = mi;
         }
-        gather_collide_cell<GC_BATCH>(a, c, m, M, lx, ly, lz, mark, K);
+        const bool write_back = gc_gather_collide<GC_BATCH>(a, c, m, M, P, lx, ly, lz, mark, K);
+        const GcMeta mc = m;
+        // the next cell: its first batch and keys go out before this cell's
+        // write-back stores, so waiting for them does not wait for those
         if (++j == a.chunk) {
             cur = nxt;
             q0 = q1;
@@ -1121,7 +1160,14 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
             nxt = load_gc_batch(a, q1);
             j = 0;
         }
+        c64 = (uint64_t)q0 * a.chunk + j;
+        if (c64 < a.ncells) {
+            DSMC_GC_META(j);
+            gc_load_ahead<GC_BATCH>(a, m, P);
+        }
+        if (write_back) gc_write_back(a, c, mc, lx, ly, lz, K);
     }
+#undef DSMC_GC_META
 }
 
 // Collision, candidate and overflow totals of a fused gather + collide pass
