@@ -188,7 +188,7 @@ def pmc_pass(args):
 
 
 def _short(name):
-    m = re.search(r"\b(k_\w+)\(", name)
+    m = re.search(r"\b(k_\w+)(?:<[^>]*>)?\(", name)
     return m.group(1) if m else name
 
 

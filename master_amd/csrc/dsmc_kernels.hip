@@ -4,13 +4,15 @@
 //   k_move            accelerate + move + walls + periodic, in place, in the
 //                     current cell-sorted order; classifies every particle as
 //                     staying in its cell or moving to another (mover list)
+//                     (k_move_overflow: particles beyond a cell's queue)
 //   k_scan_*          new cell counts and their exclusive scans
 //   k_bucket_cells    movers into per-destination-cell buckets
-//   k_gather          per destination cell: stable gather (arrivals from lower
-//                     old indices, stayers, arrivals from higher ones) into
-//                     the other state buffer
-//   k_collide         per cell: NTC collisions on the cell's velocities in
-//                     LDS, per-cell FP64 moments, write-back
+//   k_gather_collide  per destination cell: stable gather (arrivals from
+//                     lower old indices, stayers, arrivals from higher ones)
+//                     into the other state buffer, NTC collisions on the
+//                     cell's velocities in LDS, per-cell FP64 moments,
+//                     write-back; k_cell_totals sums its statistics
+//   k_gather, k_collide   the same as two passes (the unfused option)
 // The stable order is the one of a stable counting sort keyed by cell
 // (matlab/3d/sortParticles.m:11-63): inside a cell, particles keep the order
 // of their indices before the step.
@@ -855,8 +857,8 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
 }
 
 // ------------------------------------------------------------------ gather + collide
-// The stable gather of k_gather and the collide of k_collide in one wave per
-// destination cell: positions and voxels go to the other state buffer as in
+// The stable gather of k_gather and the collide of k_collide, one destination
+// cell at a time per wave: positions and voxels go to the other state buffer as in
 // k_gather, velocities into LDS at their final place in the cell, where the
 // NTC loop (ntc_cell) and the FP64 moments run before the velocities are
 // written once.  This saves the collide pass's velocity re-read and write
