@@ -33,7 +33,16 @@ typedef enum {
     DSMC_ERR_NUMERIC = 7        /* device error counters non-zero after a step */
 } dsmc_status;
 
-typedef enum { DSMC_WALL_THERMAL = 0, DSMC_WALL_SPECULAR = 1 } dsmc_wall_model;
+/* surface_interaction_model (system.cpp:343-353): thermal (colliderthermal.cpp),
+ * specular (colliderspecular.cpp), cercignani_lampis
+ * (collidercercignanilampis.cpp:13-45 with random.cpp:70-91).  The
+ * reference's maxwell collider only prints that it is not implemented
+ * (collidermaxwell.cpp:13-16) and is rejected. */
+typedef enum {
+    DSMC_WALL_THERMAL = 0,
+    DSMC_WALL_SPECULAR = 1,
+    DSMC_WALL_CERCIGNANI_LAMPIS = 2
+} dsmc_wall_model;
 
 /* Settings (settings.h:14-47) after System::initialize's unit conversion
  * (system.cpp:253-266): every field is in reduced units. */
@@ -52,6 +61,10 @@ typedef struct {
     int seed;                       /* settings seed                         */
     int wall_model;                 /* settings surface_interaction_model    */
     int64_t max_molecules;          /* settings max_molecules_per_node (0: no cap) */
+    double alpha_n, alpha_t;        /* settings alpha_n, alpha_t (settings.cpp:40-41): the
+                                     * Cercignani-Lampis normal energy and tangential
+                                     * momentum accommodation, 0 < alpha_n <= 1,
+                                     * 0 <= alpha_t <= 1 (read for that model only) */
 } dsmc_settings;
 
 /* Voxel world (the periodic grid a one-rank world file tiles, dsmc_io.cpp:183-220):
@@ -127,7 +140,8 @@ dsmc_status dsmc_step(dsmc_handle *h, int nsteps);
 dsmc_status dsmc_synchronize(dsmc_handle *h);
 dsmc_status dsmc_get_counters(dsmc_handle *h, dsmc_counters *c);
 /* The reference chooses the collider from surface_interaction_model
- * (system.cpp:343-353). */
+ * (system.cpp:343-353); the Cercignani-Lampis model uses the settings'
+ * alpha_n and alpha_t. */
 dsmc_status dsmc_set_wall_model(dsmc_handle *h, int model);
 
 /* --- sampling (per-cell analogue of StatisticsSampler, statisticssampler.cpp:42-65) */

@@ -18,8 +18,8 @@ namespace dsmc {
 
 enum : uint32_t { V_EMPTY = 0, V_WALL = 1, V_BOUNDARY = 2 };
 enum : uint32_t { TAG_INIT_POS = 1, TAG_INIT_VEL = 2, TAG_WALL = 3, TAG_RESEED = 4,
-                  TAG_SELECT = 5, TAG_SCATTER = 6 };
-enum : int { WALL_THERMAL = 0, WALL_SPECULAR = 1 };
+                  TAG_SELECT = 5, TAG_SCATTER = 6, TAG_CL_NORMAL = 9, TAG_CL_TANGENT = 10 };
+enum : int { WALL_THERMAL = 0, WALL_SPECULAR = 1, WALL_CERCIGNANI_LAMPIS = 2 };
 
 // Per ensemble member constants.
 struct Member {
@@ -41,6 +41,14 @@ struct Member {
     int wall_model;
     uint32_t key_lo, key_hi;
     uint32_t pfirst;     // first particle of the member in cell order (members keep their particles)
+    // Cercignani-Lampis wall (collidercercignanilampis.cpp:15-24, random.cpp:10-16), FP64 as there
+    double cl_factor;    // sqrt(2) sqrt(T_wall / m)
+    double cl_sqrt1ma;   // sqrt(1 - alpha_n)
+    double cl_sd;        // sqrt(alpha_n / 2)
+    double cl_argk;      // 2 sqrt(1 - alpha_n) / alpha_n
+    double cl_log_bmax;  // log(velocity_from_SI(1500 m/s) / cl_factor)
+    double cl_ta;        // sqrt(alpha_t) sqrt(T_wall / m)
+    double cl_tb;        // 1 - sigma, sigma = 1 - sqrt(1 - alpha_t)
 };
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
@@ -233,20 +241,102 @@ __device__ __forceinline__ bool reseed(const Member &M, const uint8_t *types, ui
     return false;
 }
 
-// Surface interaction (colliderthermal.cpp:13-46, colliderspecular.cpp:8-32).
-// frame = 9 floats: normal, tangent1, tangent2.
-__device__ __forceinline__ void wall_collide(const Member &M, const float *frame, uint32_t step,
+// log(exp(-x) I0(x)) with I0 from the polynomial approximations of
+// Abramowitz & Stegun 9.8.1 (|x| < 3.75) and 9.8.2 (|x| >= 3.75), the ones
+// Random::bessel_i0 evaluates (random.cpp:27-41); in logs, since
+// exp(-x) I0(x) overflows for the large negative x of fast incoming particles.
+__device__ __forceinline__ double log_exp_i0(double x) {
+    const double ax = fabs(x);
+    if (ax < 3.75) {
+        double y = x / 3.75;
+        y *= y;
+        const double i0 = 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 +
+                          y * (0.360768e-1 + y * 0.45813e-2)))));
+        return log(i0) - x;
+    }
+    const double y = 3.75 / ax;
+    const double p = 0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 + y * (0.916281e-2 +
+                     y * (-0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2)))))));
+    return ax - 0.5 * log(ax) + log(p) - x;
+}
+
+// Draw cap of the Cercignani-Lampis normal component (the reference loops
+// without one); reached, the last proposal is kept and counted as an error.
+static constexpr uint32_t CL_MAX_TRIALS = 1u << 17;
+
+// Cercignani-Lampis normal component in units of cl_factor
+// (Random::next_cercignani_lampis_normal_component, random.cpp:70-91): a
+// proposal v ~ N(mu, sd^2), mu = sqrt(1 - alpha_n) w_in, is kept when
+// b = v exp(-arg) I0(arg) / b_max >= U, arg = cl_argk w_in v.  Proposals with
+// v <= 0 always fail there (b < 0 <= U), so they are drawn here from the
+// normal truncated to v > 0, which leaves the accepted distribution as it is:
+// plainly by rejection when mu >= 0, else by Robert's exponential rejection
+// (Statistics and Computing 5:121, 1995).  An incoming particle has w_in < 0,
+// mu < 0, where the plain loop can need 1e8 draws.  Draw t of wall hit `hit`
+// is Philox (TAG_CL_NORMAL, step, pidx, hit | t << 8).
+__device__ __forceinline__ double cl_normal(const Member &M, uint32_t step, uint32_t pidx, uint32_t hit,
+                                            double w_in, bool &ok) {
+    const double mu = M.cl_sqrt1ma * w_in, sd = M.cl_sd;
+    const double a = -mu / sd;                      // truncation point in sd units
+    const double lam = 0.5 * (a + sqrt(a * a + 4.0));
+    double v = 0.0;
+    for (uint32_t t = 0; t < CL_MAX_TRIALS; t++) {
+        const uint4 o = philox(TAG_CL_NORMAL, step, pidx, hit | (t << 8), M.key_lo, M.key_hi);
+        double z;
+        if (a <= 0.0) {
+            z = sqrt(-2.0 * log(u01d(o.x))) * cospi(2.0 * u01d(o.y));
+            if (z <= a) continue;
+        } else {
+            z = a - log(u01d(o.x)) / lam;
+            if (log(u01d(o.y)) > -0.5 * (z - lam) * (z - lam)) continue;
+        }
+        v = mu + sd * z;
+        if (!(v > 0.0)) continue;
+        const double log_b = log(v) + log_exp_i0(M.cl_argk * w_in * v) - M.cl_log_bmax;
+        if (log(u01d(o.z)) <= log_b) return v;
+    }
+    ok = false;
+    return v;
+}
+
+// Surface interaction (colliderthermal.cpp:13-46, colliderspecular.cpp:8-32,
+// collidercercignanilampis.cpp:13-45).  frame = 9 floats: normal, tangent1,
+// tangent2.  Returns false when the Cercignani-Lampis sampler hit its cap.
+// CL: compile the Cercignani-Lampis model in (its FP64 sampler needs far
+// more registers than the other models; kernels instantiate both and run
+// this one only for handles that use the model).
+template <bool CL>
+__device__ __forceinline__ bool wall_collide(const Member &M, const float *frame, uint32_t step,
                                              uint32_t pidx, uint32_t hit, float &vx, float &vy,
                                              float &vz) {
     float nx = frame[0], ny = frame[1], nz = frame[2];
     float ax = frame[3], ay = frame[4], az = frame[5];
     float bx = frame[6], by = frame[7], bz = frame[8];
     float vn, vt1, vt2;
+    bool ok = true;
     if (M.wall_model == WALL_SPECULAR) {
         float un = nx * vx + ny * vy + nz * vz;
         float u1 = ax * vx + ay * vy + az * vz;
         float u2 = bx * vx + by * vy + bz * vz;
         vn = -un; vt1 = u1; vt2 = u2;
+    } else if (CL && M.wall_model == WALL_CERCIGNANI_LAMPIS) {
+        // components in FP64 as the reference's (double velocity, float frame)
+        const double dvx = vx, dvy = vy, dvz = vz;
+        const double un = dvx * nx + dvy * ny + dvz * nz;
+        const double u1 = dvx * ax + dvy * ay + dvz * az;
+        const double u2 = dvx * bx + dvy * by + dvz * bz;
+        const double w = cl_normal(M, step, pidx, hit, un / M.cl_factor, ok);
+        const uint4 o = philox(TAG_CL_TANGENT, step, pidx, hit, M.key_lo, M.key_hi);
+        const double rr = sqrt(-2.0 * log(u01d(o.x)));
+        double sp, cp;
+        sincospi(2.0 * u01d(o.y), &sp, &cp);
+        const double dn = w * M.cl_factor;
+        const double d1 = rr * cp * M.cl_ta + u1 * M.cl_tb;
+        const double d2 = rr * sp * M.cl_ta + u2 * M.cl_tb;
+        vx = (float)(dn * nx + d1 * ax + d2 * bx);
+        vy = (float)(dn * ny + d1 * ay + d2 * by);
+        vz = (float)(dn * nz + d1 * az + d2 * bz);
+        return ok;
     } else {
         uint4 o = philox(TAG_WALL, step, pidx, hit, M.key_lo, M.key_hi);
         float rr = sqrtf(-2.0f * logf(u01f(o.y)));
@@ -259,6 +349,7 @@ __device__ __forceinline__ void wall_collide(const Member &M, const float *frame
     vx = vn * nx + vt1 * ax + vt2 * bx;
     vy = vn * ny + vt1 * ay + vt2 * by;
     vz = vn * nz + vt1 * az + vt2 * bz;
+    return ok;
 }
 
 // One particle's step of System::accelerate (system.cpp:133-144) and
@@ -272,7 +363,7 @@ __device__ __forceinline__ void wall_collide(const Member &M, const float *frame
 // types, frames) and mi is the member: the path reads its descriptor and
 // tables itself, behind a compiler barrier, so none of them is hoisted out
 // of the caller's loop to stay in registers for every particle.
-template <class ArgsPtr>
+template <bool CL, class ArgsPtr>
 __device__ __forceinline__ int32_t move_general(ArgsPtr ap, uint32_t mi, const VoxelView &V,
                                                 uint32_t step, uint32_t pidx, int &ix, int &iy,
                                                 int &iz, float &ox, float &oy, float &oz, float &vx,
@@ -357,7 +448,7 @@ __device__ __forceinline__ int32_t move_general(ArgsPtr ap, uint32_t mi, const V
             dt = dt - tau;
             int gx = wrapi(bx, M.n[0]), gy = wrapi(by, M.n[1]), gz = wrapi(bz, M.n[2]);
             size_t lin = ((size_t)gx * M.n[1] + gy) * M.n[2] + gz;
-            wall_collide(M, frames + 9 * lin, step, pidx, hit, vx, vy, vz);
+            if (!wall_collide<CL>(M, frames + 9 * lin, step, pidx, hit, vx, vy, vz)) cnt.errors++;
             hit++;
             cnt.wall_hits++;
             last = (int32_t)lin;

@@ -130,6 +130,24 @@ bool cells_valid(const dsmc_settings &s, const uint32_t n[3]) {
     return true;
 }
 
+
+// Cercignani-Lampis constants (collidercercignanilampis.cpp:15-24 and the
+// Random constructor, random.cpp:10-16), FP64 as there.  v_max is
+// UnitConverter::velocity_from_SI(1500) (collidercercignanilampis.cpp:21),
+// v0 = L0 / t0 = sqrt(E0 / m0) (unitconverter.cpp:5-15).
+static void set_cl_constants(Member &d, const dsmc_settings &s) {
+    const double sw = std::sqrt(s.wall_temperature / s.mass);
+    const double an = s.alpha_n > 0.0 ? s.alpha_n : 1.0, at = s.alpha_t;
+    const double v0 = std::sqrt(1.65088e-21 / 1.66053886e-27);
+    d.cl_factor = std::sqrt(2.0) * sw;
+    d.cl_sqrt1ma = std::sqrt(1.0 - an);
+    d.cl_sd = std::sqrt(an / 2.0);
+    d.cl_argk = 2.0 * (std::sqrt(1.0 - an) / an);
+    d.cl_log_bmax = std::log((1500.0 / v0) / d.cl_factor);
+    d.cl_ta = std::sqrt(at) * sw;
+    d.cl_tb = 1.0 - (1.0 - std::sqrt(1.0 - at));
+}
+
 dsmc_status build_member(dsmc_handle *h, const dsmc_settings &s, const dsmc_world &w, MemberHost &m) {
     if (!w.types || !w.normals || !w.tangents1 || !w.tangents2)
         return fail(h, DSMC_ERR_INVALID_ARGUMENT, "world arrays missing");
@@ -189,6 +207,10 @@ dsmc_status build_member(dsmc_handle *h, const dsmc_settings &s, const dsmc_worl
     d.s_wall = (float)std::sqrt(s.wall_temperature / s.mass);
     d.s_gas = (float)std::sqrt(s.temperature / s.mass);
     d.wall_model = s.wall_model;
+    if (s.wall_model == DSMC_WALL_CERCIGNANI_LAMPIS &&
+        !(s.alpha_n > 0.0 && s.alpha_n <= 1.0 && s.alpha_t >= 0.0 && s.alpha_t <= 1.0))
+        return fail(h, DSMC_ERR_CONFIG, "cercignani_lampis needs 0 < alpha_n <= 1 and 0 <= alpha_t <= 1");
+    set_cl_constants(d, s);
     uint64_t key = (uint64_t)(int64_t)s.seed;
     d.key_lo = (uint32_t)key;
     d.key_hi = (uint32_t)(key >> 32);
@@ -521,6 +543,8 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
         m.overflow = h->d_bucket;
         m.n_overflow = h->d_mvcount + 2;
         m.slowq_cap = h->move_queue;
+        m.any_cl = 0;
+        for (auto &mb : h->mem) m.any_cl |= mb.dev.wall_model == DSMC_WALL_CERCIGNANI_LAMPIS;
         m.cell_begin = 0;
         m.ncells = h->ncells;
         Scope sc(h, "k_move");
@@ -1008,7 +1032,12 @@ dsmc_status dsmc_get_counters(dsmc_handle *h, dsmc_counters *c) {
 }
 
 dsmc_status dsmc_set_wall_model(dsmc_handle *h, int model) {
-    if (!h || (model != DSMC_WALL_THERMAL && model != DSMC_WALL_SPECULAR)) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h || (model != DSMC_WALL_THERMAL && model != DSMC_WALL_SPECULAR && model != DSMC_WALL_CERCIGNANI_LAMPIS))
+        return DSMC_ERR_INVALID_ARGUMENT;
+    if (model == DSMC_WALL_CERCIGNANI_LAMPIS)
+        for (auto &m : h->mem)
+            if (!(m.s.alpha_n > 0.0 && m.s.alpha_n <= 1.0 && m.s.alpha_t >= 0.0 && m.s.alpha_t <= 1.0))
+                return fail(h, DSMC_ERR_CONFIG, "cercignani_lampis needs 0 < alpha_n <= 1 and 0 <= alpha_t <= 1");
     std::vector<Member> ms;
     for (auto &m : h->mem) {
         m.dev.wall_model = model;
@@ -1017,6 +1046,7 @@ dsmc_status dsmc_set_wall_model(dsmc_handle *h, int model) {
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(h->d_members, ms.data(), ms.size() * sizeof(Member), hipMemcpyHostToDevice));
+    drop_graphs(h);     // the mover's instantiation depends on the model
     return DSMC_OK;
 }
 

@@ -118,9 +118,16 @@ extern "C" dsmc_status dsmc_settings_from_ini(const char *path, dsmc_settings *o
 #undef REQ
     const std::string *model = find(items, "surface_interaction_model");
     if (!model) return DSMC_ERR_CONFIG;
+    s.alpha_n = s.alpha_t = 1.0;
     if (*model == "thermal") s.wall_model = DSMC_WALL_THERMAL;
     else if (*model == "specular") s.wall_model = DSMC_WALL_SPECULAR;
-    else return DSMC_ERR_CONFIG;   // cercignani_lampis / maxwell: not on this path
+    else if (*model == "cercignani_lampis") {
+        s.wall_model = DSMC_WALL_CERCIGNANI_LAMPIS;
+        if (!getd("alpha_n", v)) return DSMC_ERR_CONFIG;
+        s.alpha_n = v;
+        if (!getd("alpha_t", v)) return DSMC_ERR_CONFIG;
+        s.alpha_t = v;
+    } else return DSMC_ERR_CONFIG;   // maxwell: unimplemented in the reference (collidermaxwell.cpp:13-16)
     if (world_dir && world_dir_len > 0) {
         const std::string *w = find(items, "world");
         std::snprintf(world_dir, (size_t)world_dir_len, "%s", w ? w->c_str() : "");

@@ -34,6 +34,7 @@ struct MoveArgs {
     uint32_t *overflow, *n_overflow;    // particles beyond a cell's queue (n entries), their count
     uint32_t cell_begin, ncells;
     uint32_t slowq_cap;                 // queue entries in use, <= MOVE_SLOWQ
+    int any_cl;                         // some member uses the Cercignani-Lampis wall model
 };
 
 struct ClassifyArgs {

@@ -69,7 +69,8 @@ __device__ __forceinline__ void load_tile(const Member &M, const uint8_t *types,
 #endif
 // REC: also record each particle's wall-hit count and last wall voxel (a
 // test hook; the production instance carries neither pointer)
-template <bool REC>
+// CL: with the Cercignani-Lampis wall model compiled in (wall_collide).
+template <bool REC, bool CL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WAVES_PER_EU, 8))) void k_move(MoveArgs a) {
     const uint32_t c = blockIdx.x + a.cell_begin;
     const uint32_t lane = threadIdx.x;
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
             uint32_t nhit = 0;
             // draws are keyed by the index inside the member, so a member of
             // an ensemble steps exactly as it would alone
-            const int32_t last = move_general(kargs, mi, V, step, q - M.pfirst, ix, iy, iz, qox, qoy, qoz,
+            const int32_t last = move_general<CL>(kargs, mi, V, step, q - M.pfirst, ix, iy, iz, qox, qoy, qoz,
                                               qvx, qvy, qvz, cnt, nhit);
             at_off(a.ox, qo) = qox; at_off(a.oy, qo) = qoy; at_off(a.oz, qo) = qoz;
             at_off(a.vx, qo) = qvx; at_off(a.vy, qo) = qvy; at_off(a.vz, qo) = qvz;
@@ -234,6 +235,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
 // Particles whose cell's queue was full (k_move), one per lane: the cell
 // from the start array, the move as in k_move's deferred loop, the mover
 // slot from an atomic on the cell's leaver count (k_move has written it).
+template <bool CL>
 __global__ __launch_bounds__(64) void k_move_overflow(MoveArgs a) {
     const uint32_t n = *a.n_overflow;
     const uint32_t step = *a.step;
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(64) void k_move_overflow(MoveArgs a) {
         unpack_vox(a.vox[q], ix, iy, iz);
         MoveCounters cnt = {0, 0, 0, {0, 0, 0}};
         uint32_t nhit = 0;
-        const int32_t last = move_general(kargs, mi, V, step, q - M.pfirst, ix, iy, iz, qox, qoy, qoz, qvx,
+        const int32_t last = move_general<CL>(kargs, mi, V, step, q - M.pfirst, ix, iy, iz, qox, qoy, qoz, qvx,
                                           qvy, qvz, cnt, nhit);
         a.ox[q] = qox; a.oy[q] = qoy; a.oz[q] = qoz;
         a.vx[q] = qvx; a.vy[q] = qvy; a.vz[q] = qvz;
@@ -1327,12 +1329,21 @@ __global__ __launch_bounds__(256) void k_totals_partial(const float *vx, const f
 
 hipError_t launch_move(const MoveArgs &a, uint32_t ncells, hipStream_t s) {
     if (ncells == 0) return hipSuccess;
-    if (a.rec_nhit)
-        hipLaunchKernelGGL(k_move<true>, dim3(ncells), dim3(64), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_move<false>, dim3(ncells), dim3(64), 0, s, a);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_move_overflow, dim3(MOVE_OVERFLOW_BLOCKS), dim3(64), 0, s, a);
+    if (a.any_cl) {
+        if (a.rec_nhit)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move<true, true>), dim3(ncells), dim3(64), 0, s, a);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move<false, true>), dim3(ncells), dim3(64), 0, s, a);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_move_overflow<true>, dim3(MOVE_OVERFLOW_BLOCKS), dim3(64), 0, s, a);
+    } else {
+        if (a.rec_nhit)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move<true, false>), dim3(ncells), dim3(64), 0, s, a);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move<false, false>), dim3(ncells), dim3(64), 0, s, a);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_move_overflow<false>, dim3(MOVE_OVERFLOW_BLOCKS), dim3(64), 0, s, a);
+    }
     LAUNCH_CHECK();
     return hipSuccess;
 }

@@ -22,7 +22,7 @@ P = ctypes.c_void_p
 D = ctypes.c_double
 I64 = ctypes.c_int64
 
-WALL_THERMAL, WALL_SPECULAR = 0, 1
+WALL_THERMAL, WALL_SPECULAR, WALL_CERCIGNANI_LAMPIS = 0, 1, 2
 OPT_PHASES, OPT_RECORD, OPT_TOTALS, OPT_LDS_CAPACITY, OPT_USE_GRAPH, OPT_TIMING = 1, 2, 3, 4, 5, 6
 OPT_COLLIDE_CHUNK, OPT_GRAPH_STEPS, OPT_FUSED, OPT_MOVE_QUEUE = 7, 8, 9, 10
 PHASE_MOVE, PHASE_SORT, PHASE_COLLIDE = 1, 2, 4
@@ -44,7 +44,7 @@ class Settings(ctypes.Structure):
                 ("dt", D), ("mass", D), ("temperature", D), ("wall_temperature", D), ("gravity", D),
                 ("flow_direction", ctypes.c_int), ("density", D), ("diam", D),
                 ("atoms_per_molecule", ctypes.c_int), ("seed", ctypes.c_int), ("wall_model", ctypes.c_int),
-                ("max_molecules", I64)]
+                ("max_molecules", I64), ("alpha_n", D), ("alpha_t", D)]
 
 
 class World(ctypes.Structure):
@@ -118,9 +118,14 @@ def settings_from_ini(ini: dict) -> Settings:
     s.atoms_per_molecule = int(float(ini["atoms_per_molecule"]))
     s.seed = int(float(ini["seed"]))
     model = ini["surface_interaction_model"]
-    if model not in ("thermal", "specular"):
+    models = {"thermal": WALL_THERMAL, "specular": WALL_SPECULAR, "cercignani_lampis": WALL_CERCIGNANI_LAMPIS}
+    if model not in models:
+        # maxwell: unimplemented in the reference (collidermaxwell.cpp:13-16)
         raise DsmcError(f"surface_interaction_model {model!r} is not on this path")
-    s.wall_model = WALL_SPECULAR if model == "specular" else WALL_THERMAL
+    s.wall_model = models[model]
+    s.alpha_n, s.alpha_t = 1.0, 1.0
+    if model == "cercignani_lampis":
+        s.alpha_n, s.alpha_t = float(ini["alpha_n"]), float(ini["alpha_t"])
     s.max_molecules = int(float(ini.get("max_molecules_per_node", 0)))
     return s
 

@@ -43,6 +43,9 @@ def lib():
         _lib.or_cell_index_from_position.restype = L
         _lib.or_geom_porosity.restype = ctypes.c_float
         _lib.or_system_wall_log.restype = L
+        _lib.or_cl_log_exp_i0.restype = D
+        _lib.or_cl_log_exp_i0.argtypes = [D]
+        _lib.or_cl_normal_samples.restype = L
     return _lib
 
 
@@ -60,6 +63,7 @@ def ref():
         _ref.ref_sys_create.restype = P
         _ref.ref_sys_num.restype = L
         _ref.ref_sys_log_size.restype = L
+        _ref.ref_cl_normal_samples.restype = L
         for f in ("ref_sys_num", "ref_sys_get_state", "ref_sys_set_state", "ref_sys_move_all",
                   "ref_sys_step", "ref_sys_counters", "ref_sys_log_size", "ref_sys_log",
                   "ref_sys_log_clear", "ref_sys_num_active", "ref_sys_cells", "ref_sys_cell_volumes"):
@@ -76,6 +80,27 @@ def ran1_sequence(seed, nu, ng, use_ref=False):
     else:
         lib().or_ran1_sequence(L(seed), nu, _ptr(u), ng, _ptr(g))
     return u[:nu], g[:ng]
+
+
+# ---------------------------------------------------------------- Cercignani-Lampis
+def cl_log_exp_i0(x):
+    return lib().or_cl_log_exp_i0(float(x))
+
+
+def cl_normal_samples(alpha_n, w_in, bmax, n, rng_mode=None, seed=-1, key=1, step=1, hit=0, use_ref=False):
+    """n Cercignani-Lampis normal components (units of sqrt(2 T_wall / m)) for
+    an incoming normal velocity w_in: the oracle's ran1 restatement of the
+    reference loop (rng_mode RAN1), the oracle's truncated-proposal sampler
+    (PHILOX, the default), or the compiled reference's own loop (use_ref).
+    Returns (samples, proposals drawn)."""
+    out = np.zeros(n)
+    if use_ref:
+        t = ref().ref_cl_normal_samples(L(seed), D(alpha_n), D(w_in), D(bmax), n, _ptr(out))
+        return out, t
+    mode = PHILOX if rng_mode is None else rng_mode
+    t = lib().or_cl_normal_samples(mode, L(seed), ctypes.c_uint64(key), ctypes.c_uint32(step), ctypes.c_uint32(hit),
+                                   D(alpha_n), D(w_in), D(np.log(bmax)), n, _ptr(out))
+    return out, t
 
 
 def philox(ctr, key):
@@ -172,6 +197,7 @@ class Params(ctypes.Structure):
         ("gravity", D), ("flow_direction", ctypes.c_int), ("density", D), ("diam", D),
         ("atoms_per_molecule", D), ("wall_model", ctypes.c_int), ("rng_mode", ctypes.c_int),
         ("seed", L), ("key", ctypes.c_uint64), ("porosity", ctypes.c_float),
+        ("alpha_n", D), ("alpha_t", D),
     ]
 
 
@@ -182,7 +208,8 @@ class Counters(ctypes.Structure):
 
 
 RAN1, PHILOX = 0, 1
-THERMAL, SPECULAR = 0, 1
+THERMAL, SPECULAR, CERCIGNANI_LAMPIS = 0, 1, 2
+_WALLS = {"thermal": THERMAL, "specular": SPECULAR, "cercignani_lampis": CERCIGNANI_LAMPIS}
 
 
 def params_from_settings(ini, world, rng_mode=PHILOX, key=None):
@@ -202,7 +229,9 @@ def params_from_settings(ini, world, rng_mode=PHILOX, key=None):
     p.density = float(ini["density"]) / u["number_density0"]
     p.diam = float(ini["diam"])
     p.atoms_per_molecule = float(int(float(ini["atoms_per_molecule"])))
-    p.wall_model = SPECULAR if ini["surface_interaction_model"] == "specular" else THERMAL
+    p.wall_model = _WALLS[ini["surface_interaction_model"]]
+    p.alpha_n = float(ini.get("alpha_n", 1.0))
+    p.alpha_t = float(ini.get("alpha_t", 1.0))
     p.rng_mode = rng_mode
     p.seed = int(float(ini["seed"]))
     p.key = int(key if key is not None else int(float(ini["seed"])))

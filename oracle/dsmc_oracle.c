@@ -135,7 +135,7 @@ double or_u01(uint32_t x) {
 
 /* Counter layout shared with the GPU kernels: ctr = {tag, step, index, sub}. */
 enum { TAG_INIT_POS = 1, TAG_INIT_VEL = 2, TAG_WALL = 3, TAG_RESEED = 4,
-       TAG_SELECT = 5, TAG_SCATTER = 6 };
+       TAG_SELECT = 5, TAG_SCATTER = 6, TAG_CL_NORMAL = 9, TAG_CL_TANGENT = 10 };
 
 static void philox_draw(uint64_t key, uint32_t tag, uint32_t step, uint32_t index,
                         uint32_t sub, uint32_t out[4]) {
@@ -612,6 +612,101 @@ static void find_position(or_system *s, long index, int tag_step) {
 }
 
 /* ---------- wall models ---------- */
+/* ---- Cercignani-Lampis (collidercercignanilampis.cpp:13-45, random.cpp:27-91) */
+
+/* Random::bessel_i0 (random.cpp:27-41): the polynomial approximations of
+ * Abramowitz & Stegun 9.8.1 and 9.8.2. */
+static double cl_bessel_i0(double x) {
+    double ax = fabs(x), y;
+    if (ax < 3.75) {
+        y = x / 3.75;
+        y *= y;
+        return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 +
+               y * (0.360768e-1 + y * 0.45813e-2)))));
+    }
+    y = 3.75 / ax;
+    return (exp(ax) / sqrt(ax)) * (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 +
+           y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+}
+
+/* log(exp(-x) I0(x)) from the same polynomials, without overflow. */
+double or_cl_log_exp_i0(double x);
+static double cl_log_exp_i0(double x) { return or_cl_log_exp_i0(x); }
+double or_cl_log_exp_i0(double x) {
+    double ax = fabs(x);
+    if (ax < 3.75) return log(cl_bessel_i0(x)) - x;
+    double y = 3.75 / ax;
+    double p = 0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 + y * (0.916281e-2 +
+               y * (-0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2)))))));
+    return ax - 0.5 * log(ax) + log(p) - x;
+}
+
+#define OR_CL_MAX_TRIALS (1u << 17)
+
+/* The reference's loop (random.cpp:70-91) in units of factor: v ~ N(mu, sd^2)
+ * with mu = sqrt(1 - alpha_n) w_in, kept when v exp(-arg) I0(arg) / b_max >= U.
+ * The gauss draws two uniforms, the acceptance one (next_gauss, then
+ * next_double, as written there). */
+static double cl_normal_ran1(or_ran1_t *r, double alpha_n, double w_in, double bmax, long *trials) {
+    double sd = sqrt(alpha_n / 2), mu = sqrt(1 - alpha_n) * w_in;
+    double k = sqrt(1 - alpha_n) / alpha_n;
+    for (;;) {
+        ++*trials;
+        double v = sd * or_ran1_gauss(r) + mu;
+        double u = or_ran1_next(r);
+        double arg = 2 * k * w_in * v;
+        double b = v * exp(-arg) * cl_bessel_i0(arg) / bmax;
+        if (b >= u) return v;
+    }
+}
+
+/* The device's sampler: the same accepted distribution from proposals
+ * truncated to v > 0 (plain rejection when mu >= 0, Robert's exponential
+ * rejection otherwise), acceptance in logs.  Draw t: Philox (TAG_CL_NORMAL,
+ * step, index, hit | t << 8).  *ok = 0 when the cap is reached. */
+static double cl_normal_philox(uint64_t key, uint32_t step, uint32_t index, uint32_t hit, double alpha_n,
+                               double w_in, double log_bmax, long *trials, int *ok) {
+    double mu = sqrt(1.0 - alpha_n) * w_in, sd = sqrt(alpha_n / 2.0);
+    double argk = 2.0 * (sqrt(1.0 - alpha_n) / alpha_n);
+    double a = -mu / sd, lam = 0.5 * (a + sqrt(a * a + 4.0)), v = 0.0;
+    for (uint32_t t = 0; t < OR_CL_MAX_TRIALS; t++) {
+        uint32_t o[4];
+        ++*trials;
+        philox_draw(key, TAG_CL_NORMAL, step, index, hit | (t << 8), o);
+        double z;
+        if (a <= 0.0) {
+            z = sqrt(-2.0 * log(or_u01(o[0]))) * cos(6.283185307179586 * or_u01(o[1]));
+            if (z <= a) continue;
+        } else {
+            z = a - log(or_u01(o[0])) / lam;
+            if (log(or_u01(o[1])) > -0.5 * (z - lam) * (z - lam)) continue;
+        }
+        v = mu + sd * z;
+        if (!(v > 0.0)) continue;
+        double log_b = log(v) + cl_log_exp_i0(argk * w_in * v) - log_bmax;
+        if (log(or_u01(o[2])) <= log_b) return v;
+    }
+    *ok = 0;
+    return v;
+}
+
+long or_cl_normal_samples(int rng_mode, long seed, uint64_t key, uint32_t step, uint32_t hit,
+                          double alpha_n, double w_in, double log_bmax, int n, double *out) {
+    long trials = 0;
+    if (rng_mode == OR_RNG_RAN1) {
+        or_ran1_t r;
+        or_ran1_seed(&r, seed);
+        for (int i = 0; i < n; i++) out[i] = cl_normal_ran1(&r, alpha_n, w_in, exp(log_bmax), &trials);
+    } else {
+        for (int i = 0; i < n; i++) {
+            int ok = 1;
+            out[i] = cl_normal_philox(key, step, (uint32_t)i, hit, alpha_n, w_in, log_bmax, &trials, &ok);
+            if (!ok) out[i] = -1.0;
+        }
+    }
+    return trials;
+}
+
 static void wall_collide(or_system *s, double *v, long cvi_halo, long index, int hit) {
     int ok;
     long g = global_from_halo(s, cvi_halo, &ok);
@@ -622,7 +717,43 @@ static void wall_collide(or_system *s, double *v, long cvi_halo, long index, int
         memcpy(s->log_vin + 3 * li, v, 3 * sizeof(double));
     }
     const float *n = s->nrm + 3 * g, *a = s->tg1 + 3 * g, *b = s->tg2 + 3 * g;
-    if (s->p.wall_model == OR_WALL_SPECULAR) {                   /* colliderspecular.cpp:8-32 */
+    if (s->p.wall_model == OR_WALL_CERCIGNANI_LAMPIS) {          /* collidercercignanilampis.cpp:13-45 */
+        double sw = s->sqrt_wall_temp_over_mass;
+        double factor = sqrt(2) * sw;
+        double v_norm_in = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+        double v_t1_in = v[0] * a[0] + v[1] * a[1] + v[2] * a[2];
+        double v_t2_in = v[0] * b[0] + v[1] * b[1] + v[2] * b[2];
+        or_units_t u;
+        or_units(&u);
+        double v_max = 1500.0 / u.v0;                  /* velocity_from_SI(1500) */
+        double alpha_n = s->p.alpha_n, alpha_t = s->p.alpha_t;
+        double sigma = 1 - sqrt(1 - alpha_t);
+        double v_normal, v_tangent1, v_tangent2;
+        if (s->p.rng_mode == OR_RNG_RAN1) {
+            long trials = 0;
+            v_normal = cl_normal_ran1(&s->rnd, alpha_n, v_norm_in / factor, v_max / factor, &trials) * factor;
+            v_tangent1 = or_ran1_gauss(&s->rnd) * sqrt(alpha_t) * sw + v_t1_in * (1 - sigma);
+            v_tangent2 = or_ran1_gauss(&s->rnd) * sqrt(alpha_t) * sw + v_t2_in * (1 - sigma);
+        } else {
+            long trials = 0;
+            int ok = 1;
+            uint32_t o[4];
+            v_normal = cl_normal_philox(s->p.key, (uint32_t)s->cnt.steps, (uint32_t)index, (uint32_t)hit, alpha_n,
+                                        v_norm_in / factor, log(v_max / factor), &trials, &ok) * factor;
+            if (!ok) s->cnt.errors++;
+            philox_draw(s->p.key, TAG_CL_TANGENT, (uint32_t)s->cnt.steps, (uint32_t)index, (uint32_t)hit, o);
+            double rr = sqrt(-2.0 * log(or_u01(o[0])));
+            double th = 6.283185307179586 * or_u01(o[1]);
+            v_tangent1 = rr * cos(th) * (sqrt(alpha_t) * sw) + v_t1_in * (1 - sigma);
+            v_tangent2 = rr * sin(th) * (sqrt(alpha_t) * sw) + v_t2_in * (1 - sigma);
+        }
+        float n_x = n[0], n_y = n[1], n_z = n[2];
+        float t1_x = a[0], t1_y = a[1], t1_z = a[2];
+        float t2_x = b[0], t2_y = b[1], t2_z = b[2];
+        v[0] = v_normal * n_x + v_tangent1 * t1_x + v_tangent2 * t2_x;
+        v[1] = v_normal * n_y + v_tangent1 * t1_y + v_tangent2 * t2_y;
+        v[2] = v_normal * n_z + v_tangent1 * t1_z + v_tangent2 * t2_z;
+    } else if (s->p.wall_model == OR_WALL_SPECULAR) {            /* colliderspecular.cpp:8-32 */
         float n_x = n[0], n_y = n[1], n_z = n[2];
         float t1_x = a[0], t1_y = a[1], t1_z = a[2];
         float t2_x = b[0], t2_y = b[1], t2_z = b[2];

@@ -93,7 +93,7 @@ int or_geom_poly_spheres_mask(int nx, int ny, int nz, double rmin_vox, double rm
 
 /* ---------------- system ---------------- */
 enum { OR_RNG_RAN1 = 0, OR_RNG_PHILOX = 1 };
-enum { OR_WALL_THERMAL = 0, OR_WALL_SPECULAR = 1 };
+enum { OR_WALL_THERMAL = 0, OR_WALL_SPECULAR = 1, OR_WALL_CERCIGNANI_LAMPIS = 2 };
 
 typedef struct {
     int nx, ny, nz;                 /* global voxel counts            */
@@ -113,6 +113,7 @@ typedef struct {
     long seed;                      /* settings->seed                 */
     uint64_t key;                   /* Philox key (counter mode)      */
     float porosity;                 /* local porosity from world file */
+    double alpha_n, alpha_t;        /* Cercignani-Lampis accommodation (settings.cpp:40-41) */
 } or_params_t;
 
 typedef struct {
@@ -127,6 +128,17 @@ typedef struct {
 } or_counters_t;
 
 typedef struct or_system or_system;
+
+/* Cercignani-Lampis normal component of an incoming normal velocity w_in (in
+ * units of factor = sqrt(2 T_wall / m)), n samples.  rng_mode OR_RNG_RAN1:
+ * the reference's loop (random.cpp:70-91) on ran1 seeded with `seed`;
+ * OR_RNG_PHILOX: the truncated-proposal sampler of the device, draws keyed by
+ * (key, step, index = sample number, hit).  log_bmax = log(v_max / factor).
+ * Returns the number of proposals drawn. */
+/* log(exp(-x) I0(x)) from Random::bessel_i0's polynomials (random.cpp:27-41). */
+double or_cl_log_exp_i0(double x);
+long or_cl_normal_samples(int rng_mode, long seed, uint64_t key, uint32_t step, uint32_t hit,
+                          double alpha_n, double w_in, double log_bmax, int n, double *out);
 
 or_system *or_system_create(const or_params_t *p, const uint8_t *types,
                             const float *normals, const float *t1, const float *t2);

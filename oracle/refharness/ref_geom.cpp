@@ -28,6 +28,16 @@ void ref_ran1_sequence(long seed, int nu, double *u, int ng, double *g) {
     for (int i = 0; i < ng; i++) g[i] = r.next_gauss();
 }
 
+// n draws of Random::next_cercignani_lampis_normal_component
+// (random.cpp:70-91) with factor 1: the incoming normal velocity w_in and the
+// result in units of sqrt(2 T_wall / m), bmax = v_max in the same units.
+// Returns the number of proposals the reference loop drew.
+long ref_cl_normal_samples(long seed, double alpha_n, double w_in, double bmax, int n, double *out) {
+    Random r(seed, alpha_n, 0.0);
+    for (int i = 0; i < n; i++) out[i] = r.next_cercignani_lampis_normal_component(w_in, 1.0, bmax);
+    return r.cercignani_lampis_normal_component_trials;
+}
+
 // Runs ComplexGeometry::create_packed_spheres (which ends with make_periodic
 // and calculate_normals_tangents_and_inner_points) and copies its arrays.
 int ref_packed_spheres(int nx, int ny, int nz, int spheres_type, int spheres_num,

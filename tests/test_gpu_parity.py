@@ -26,6 +26,8 @@ AMBIG = 1e-5
 def _init_pair(gpu, n_particles=200_000, wall="thermal", seed=1, gravity=None, w=None, dt_tau=0.2):
     w = w or K.world()
     ini = K.settings(w, n_particles, seed=seed, wall=wall, gravity=gravity, dt_tau=dt_tau)
+    if wall == "cercignani_lampis":
+        ini["alpha_n"], ini["alpha_t"] = 0.6, 0.4
     g = K.gpu_system(ini, w)
     g.initialize()
     st = g.state()
@@ -51,7 +53,8 @@ def test_seeding_matches_oracle(gpu):
 
 # queue 0: every particle off the mover's fast path takes its overflow pass
 @pytest.mark.parametrize("wall,mult,queue", [("thermal", 1, None), ("specular", 1, None), ("thermal", 10, None),
-                                             ("thermal", 10, 0), ("specular", 1, 5)])
+                                             ("thermal", 10, 0), ("specular", 1, 5),
+                                             ("cercignani_lampis", 1, None), ("cercignani_lampis", 10, 0)])
 def test_move_and_sort_match_oracle(gpu, wall, mult, queue):
     ini, w, g, o, st0 = _init_pair(gpu, wall=wall, gravity=0.05, dt_tau=0.2 * mult)
     if queue is not None:
