@@ -86,6 +86,31 @@ def test_settings_from_reference_ini(tmp_path):
         assert getattr(s, f) == getattr(s2, f), f
 
 
+def test_wall_models_from_reference_ini(tmp_path):
+    """surface_interaction_model (system.cpp:343-353): cercignani_lampis reads
+    alpha_n / alpha_t (settings.cpp:40-41) and needs them; maxwell, which the
+    reference only reports as unimplemented (collidermaxwell.cpp:13-16), is a
+    configuration error."""
+    w = B.packed_spheres(32, 4.0, 0.6, 1)
+    ini = C.dsmc_settings(32, w["porosity"], 1000, 0.1, 8, wall="cercignani_lampis")
+    ini["alpha_n"], ini["alpha_t"] = 0.3, 0.7
+    B.write_ini(str(tmp_path / "cl.ini"), ini)
+    s, _ = dsmc.settings_from_ini_file(str(tmp_path / "cl.ini"))
+    assert s.wall_model == dsmc.WALL_CERCIGNANI_LAMPIS and (s.alpha_n, s.alpha_t) == (0.3, 0.7)
+    s2 = dsmc.settings_from_ini(ini)
+    assert (s2.wall_model, s2.alpha_n, s2.alpha_t) == (s.wall_model, s.alpha_n, s.alpha_t)
+    del ini["alpha_t"]
+    B.write_ini(str(tmp_path / "cl2.ini"), ini)
+    with pytest.raises(dsmc.DsmcError):
+        dsmc.settings_from_ini_file(str(tmp_path / "cl2.ini"))
+    ini = dict(ini, surface_interaction_model="maxwell", alpha_t=1.0)
+    B.write_ini(str(tmp_path / "mx.ini"), ini)
+    with pytest.raises(dsmc.DsmcError):
+        dsmc.settings_from_ini_file(str(tmp_path / "mx.ini"))
+    with pytest.raises(dsmc.DsmcError):
+        dsmc.settings_from_ini(ini)
+
+
 def test_invalid_configuration_is_an_error_not_an_exit():
     w = B.packed_spheres(32, 4.0, 0.6, 1)
     ini = C.dsmc_settings(32, w["porosity"], 1000, 0.1, 8)
