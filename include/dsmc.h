@@ -144,6 +144,37 @@ dsmc_status dsmc_get_counters(dsmc_handle *h, dsmc_counters *c);
  * alpha_n and alpha_t. */
 dsmc_status dsmc_set_wall_model(dsmc_handle *h, int model);
 
+/* --- statistics (StatisticsSampler, statisticssampler.cpp:13-48, with the
+ * properties of statisticalproperty.cpp) ------------------------------------
+ * After every step whose count is a multiple of `interval` (settings
+ * statistics_interval) the device takes one sample: the kinetic energy sum
+ * (MeasureEnergy 21-40), the particles binned along y into `bins` bins
+ * (settings sampling_bins) with their counts, sums of v_z and of m v_z^2 / 2
+ * (MeasureCount 229-245, MeasureVelocityDistributionPoiseuille 262-276,
+ * MeasureTemperatureDistribution 316-330), a histogram of v_x over
+ * [-4 s, 4 s], s = sqrt(T / m) (MeasureVelocityDistribution 398-430), and
+ * count_periodic on the flow axis (MeasureNumberFlowRate 87-100).  The time
+ * series keeps the first max_samples samples; the bins sum over all samples.
+ * interval 0 switches sampling off; enabling clears the sums and starts the
+ * run's clock (System::t0).  Single-system handles only. */
+dsmc_status dsmc_set_statistics(dsmc_handle *h, int interval, int bins, int max_samples);
+/* Sums over the samples taken (each array `bins` long); any pointer may be
+ * NULL.  vz_sum and ez_sum are FP64 sums whose last bits depend on the order
+ * of the device's LDS atomics. */
+dsmc_status dsmc_get_statistics(dsmc_handle *h, int64_t *samples, uint64_t *count, double *vz_sum,
+                                double *ez_sum, uint64_t *vx_hist);
+/* The time series: n in/out (capacity in, entries out), per sample the step
+ * count, the sum of v^2 over particles (reduced units) and count_periodic on
+ * the flow axis. */
+dsmc_status dsmc_get_statistics_series(dsmc_handle *h, int64_t *n, int64_t *steps, double *v2_sum,
+                                       int64_t *periodic);
+/* The reference's statistics files into `dir` (dsmc_io.cpp:34-46 names):
+ * energy, temperature, pressure, number_flow_rate, volumetric_flow_rate
+ * (one line per sample, formats of the update functions), permeability,
+ * velocity, linear_temperature, linear_pressure and velocity_distribution
+ * (the finalize functions), SI units as there. */
+dsmc_status dsmc_write_statistics(dsmc_handle *h, const char *dir);
+
 /* --- sampling (per-cell analogue of StatisticsSampler, statisticssampler.cpp:42-65) */
 /* Accumulate per-cell moments every `every` steps (0 disables). */
 dsmc_status dsmc_set_sampling(dsmc_handle *h, int every);

@@ -89,6 +89,15 @@ struct dsmc_handle {
     hipGraph_t graph_src[2] = {nullptr, nullptr};
     bool graph_sample[2] = {false, false};
     std::vector<Timer> timers;
+    // statistics sampler (StatisticsSampler): 0 = off
+    uint32_t stats_interval = 0, stats_bins = 0, stats_cap = 0;
+    int64_t stats_step0 = 0;        // step count and time when sampling was (re)started:
+    double stats_t0 = 0;            //   System::t0 of the reference's run
+    uint32_t *d_st_pcnt = nullptr, *d_st_pvxh = nullptr, *d_st_step = nullptr;
+    double *d_st_pvz = nullptr, *d_st_pez = nullptr, *d_st_pke = nullptr;
+    unsigned long long *d_st_cnt = nullptr, *d_st_vxh = nullptr, *d_st_n = nullptr;
+    double *d_st_vz = nullptr, *d_st_ez = nullptr, *d_st_ke = nullptr;
+    long long *d_st_per = nullptr;
 };
 
 namespace {
@@ -515,6 +524,47 @@ dsmc_status sort_unsorted(dsmc_handle *h, const std::vector<uint32_t> &part_memb
     return refresh_occupancy(h);
 }
 
+StatsArgs stats_args(dsmc_handle *h) {
+    const MemberHost &m = h->mem[0];
+    StatsArgs a;
+    const int c = h->cur;
+    a.oy = h->oy[c]; a.vx = h->vx[c]; a.vy = h->vy[c]; a.vz = h->vz[c];
+    a.vox = h->vox[c];
+    a.step = h->d_step;
+    a.counters = h->d_cnt;
+    a.n = (uint32_t)h->N;
+    a.interval = h->stats_interval;
+    a.bins = h->stats_bins;
+    a.flow_axis = (m.s.flow_direction >= 0 && m.s.flow_direction < 3) ? (uint32_t)m.s.flow_direction : 3u;
+    a.series_cap = h->stats_cap;
+    a.y_h = m.s.Ly / m.dev.n[1];
+    a.y_inv_len = 1.0 / m.s.Ly;                         // grid.cpp one_over_length
+    // MeasureVelocityDistribution (statisticalproperty.cpp:398-405)
+    const double sd = std::sqrt(m.s.temperature / m.s.mass);
+    a.v_min = -4 * sd;
+    a.v_bin = (4 * sd - a.v_min) / h->stats_bins;
+    a.half_m = 0.5 * m.s.mass;
+    a.p_cnt = h->d_st_pcnt; a.p_vxh = h->d_st_pvxh;
+    a.p_vz = h->d_st_pvz; a.p_ez = h->d_st_pez; a.p_ke = h->d_st_pke;
+    a.cnt = h->d_st_cnt; a.vxh = h->d_st_vxh; a.vz_sum = h->d_st_vz; a.ez_sum = h->d_st_ez;
+    a.s_ke = h->d_st_ke; a.s_per = h->d_st_per; a.s_step = h->d_st_step;
+    a.nsamples = h->d_st_n;
+    return a;
+}
+
+void free_stats(dsmc_handle *h) {
+    dfree(h->d_st_pcnt); dfree(h->d_st_pvxh); dfree(h->d_st_step);
+    dfree(h->d_st_pvz); dfree(h->d_st_pez); dfree(h->d_st_pke);
+    dfree(h->d_st_cnt); dfree(h->d_st_vxh); dfree(h->d_st_n);
+    dfree(h->d_st_vz); dfree(h->d_st_ez); dfree(h->d_st_ke); dfree(h->d_st_per);
+    h->d_st_pcnt = h->d_st_pvxh = h->d_st_step = nullptr;
+    h->d_st_pvz = h->d_st_pez = h->d_st_pke = nullptr;
+    h->d_st_cnt = h->d_st_vxh = h->d_st_n = nullptr;
+    h->d_st_vz = h->d_st_ez = h->d_st_ke = nullptr;
+    h->d_st_per = nullptr;
+    h->stats_interval = 0;
+}
+
 dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
     hipStream_t s = h->stream;
     HIPCHK(h, launch_begin_step(h->d_step, h->d_mvcount, s));
@@ -586,6 +636,10 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
         int b = h->cur;
         HIPCHK(h, launch_totals_partial(h->vx[b], h->vy[b], h->vz[b], (uint32_t)h->N,
                                         h->d_partial + TOT_BLOCKS * 4, TOT_BLOCKS, s));
+    }
+    if (h->stats_interval) {
+        Scope sc(h, "k_stats");
+        HIPCHK(h, launch_stats(stats_args(h), s));
     }
     return DSMC_OK;
 }
@@ -720,6 +774,7 @@ void dsmc_destroy(dsmc_handle *h) {
     dfree(h->d_start[0]); dfree(h->d_start[1]);
     dfree(h->d_leave); dfree(h->d_arrive); dfree(h->d_bstart); dfree(h->d_bfill); dfree(h->d_bsums);
     dfree(h->d_step); dfree(h->d_mvcount); dfree(h->d_cnt); dfree(h->d_partial);
+    free_stats(h);
     for (auto &t : h->timers)
         for (auto &e : t.ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1053,6 +1108,148 @@ dsmc_status dsmc_set_wall_model(dsmc_handle *h, int model) {
 dsmc_status dsmc_set_sampling(dsmc_handle *h, int every) {
     if (!h || every < 0) return DSMC_ERR_INVALID_ARGUMENT;
     h->sample_every = every;
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_set_statistics(dsmc_handle *h, int interval, int bins, int max_samples) {
+    if (!h || interval < 0 || max_samples < 0) return DSMC_ERR_INVALID_ARGUMENT;
+    if (h->mem.size() != 1) return fail(h, DSMC_ERR_STATE, "statistics need a single-system handle");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drop_graphs(h);
+    free_stats(h);
+    if (interval == 0) return DSMC_OK;
+    if (bins < 1 || bins > (int)STATS_MAX_BINS) return fail(h, DSMC_ERR_INVALID_ARGUMENT, "bins must be 1..1024");
+    const size_t nb = (size_t)bins, rows = (size_t)STATS_BLOCKS * nb, cap = max_samples > 0 ? max_samples : 1;
+    HIPCHK(h, dalloc(&h->d_st_pcnt, rows)); HIPCHK(h, dalloc(&h->d_st_pvxh, rows));
+    HIPCHK(h, dalloc(&h->d_st_pvz, rows)); HIPCHK(h, dalloc(&h->d_st_pez, rows));
+    HIPCHK(h, dalloc(&h->d_st_pke, (size_t)STATS_BLOCKS));
+    HIPCHK(h, dalloc(&h->d_st_cnt, nb)); HIPCHK(h, dalloc(&h->d_st_vxh, nb));
+    HIPCHK(h, dalloc(&h->d_st_vz, nb)); HIPCHK(h, dalloc(&h->d_st_ez, nb));
+    HIPCHK(h, dalloc(&h->d_st_ke, cap)); HIPCHK(h, dalloc(&h->d_st_per, cap)); HIPCHK(h, dalloc(&h->d_st_step, cap));
+    HIPCHK(h, dalloc(&h->d_st_n, 1));
+    HIPCHK(h, hipMemset(h->d_st_cnt, 0, nb * 8)); HIPCHK(h, hipMemset(h->d_st_vxh, 0, nb * 8));
+    HIPCHK(h, hipMemset(h->d_st_vz, 0, nb * 8)); HIPCHK(h, hipMemset(h->d_st_ez, 0, nb * 8));
+    HIPCHK(h, hipMemset(h->d_st_n, 0, 8));
+    h->stats_interval = (uint32_t)interval;
+    h->stats_bins = (uint32_t)bins;
+    h->stats_cap = (uint32_t)max_samples;
+    h->stats_step0 = h->steps;
+    h->stats_t0 = h->t;
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_statistics(dsmc_handle *h, int64_t *samples, uint64_t *count, double *vz_sum,
+                                double *ez_sum, uint64_t *vx_hist) {
+    if (!h) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->stats_interval) return fail(h, DSMC_ERR_STATE, "statistics are off (dsmc_set_statistics)");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const size_t nb = h->stats_bins;
+    if (samples) {
+        unsigned long long n = 0;
+        HIPCHK(h, hipMemcpy(&n, h->d_st_n, 8, hipMemcpyDeviceToHost));
+        *samples = (int64_t)n;
+    }
+    if (count) HIPCHK(h, hipMemcpy(count, h->d_st_cnt, nb * 8, hipMemcpyDeviceToHost));
+    if (vz_sum) HIPCHK(h, hipMemcpy(vz_sum, h->d_st_vz, nb * 8, hipMemcpyDeviceToHost));
+    if (ez_sum) HIPCHK(h, hipMemcpy(ez_sum, h->d_st_ez, nb * 8, hipMemcpyDeviceToHost));
+    if (vx_hist) HIPCHK(h, hipMemcpy(vx_hist, h->d_st_vxh, nb * 8, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_statistics_series(dsmc_handle *h, int64_t *n, int64_t *steps, double *v2_sum,
+                                       int64_t *periodic) {
+    if (!h || !n || *n < 0) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->stats_interval) return fail(h, DSMC_ERR_STATE, "statistics are off (dsmc_set_statistics)");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    unsigned long long taken = 0;
+    HIPCHK(h, hipMemcpy(&taken, h->d_st_n, 8, hipMemcpyDeviceToHost));
+    const int64_t kept = (int64_t)std::min<unsigned long long>(taken, h->stats_cap);
+    const int64_t m = std::min<int64_t>(kept, *n);
+    *n = kept;
+    if (m == 0) return DSMC_OK;
+    std::vector<uint32_t> st(m);
+    std::vector<long long> per(m);
+    HIPCHK(h, hipMemcpy(st.data(), h->d_st_step, m * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(per.data(), h->d_st_per, m * 8, hipMemcpyDeviceToHost));
+    if (v2_sum) HIPCHK(h, hipMemcpy(v2_sum, h->d_st_ke, m * 8, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < m; i++) {
+        if (steps) steps[i] = st[i];
+        if (periodic) periodic[i] = per[i];
+    }
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_write_statistics(dsmc_handle *h, const char *dir) {
+    if (!h || !dir) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->stats_interval) return fail(h, DSMC_ERR_STATE, "statistics are off (dsmc_set_statistics)");
+    const MemberHost &m = h->mem[0];
+    const size_t nb = h->stats_bins;
+    int64_t samples = 0;
+    std::vector<uint64_t> cnt(nb), hist(nb);
+    std::vector<double> vz(nb), ez(nb);
+    dsmc_status st = dsmc_get_statistics(h, &samples, cnt.data(), vz.data(), ez.data(), hist.data());
+    if (st != DSMC_OK) return st;
+    int64_t ns = h->stats_cap;
+    std::vector<int64_t> steps(ns > 0 ? ns : 1), per(ns > 0 ? ns : 1);
+    std::vector<double> v2(ns > 0 ? ns : 1);
+    st = dsmc_get_statistics_series(h, &ns, steps.data(), v2.data(), per.data());
+    if (st != DSMC_OK) return st;
+    // unitconverter.cpp:3-21
+    const double m0 = 1.66053886e-27, L0 = 1e-6, E0 = 1.65088e-21, E0ev = 1.0318e-2, kb = 1.3806503e-23;
+    const double t0u = L0 * std::sqrt(m0 / E0), T0 = E0 / kb, P0 = m0 / (t0u * t0u * L0), v0 = L0 / t0u;
+    const double perm0 = L0 * L0;
+    const double mass = m.s.mass, fn = (double)m.s.atoms_per_molecule, dt = m.s.dt;
+    const double L[3] = {m.s.Lx, m.s.Ly, m.s.Lz};
+    const double volume_global = L[0] * L[1] * L[2] * (double)m.porosity;     // system.cpp:314
+    const long atoms = (long)h->N * (long)m.s.atoms_per_molecule;             // system.cpp:509-511
+    const std::string d(dir);
+    auto open = [&](const char *name) { return std::fopen((d + "/" + name).c_str(), "w"); };
+    FILE *fe = open("energy.txt"), *ft = open("temperature.txt"), *fp = open("pressure.txt");
+    FILE *fq = open("number_flow_rate.txt"), *fv = open("volumetric_flow_rate.txt");
+    FILE *fk = open("permeability.txt"), *fu = open("velocity.txt"), *flt = open("linear_temperature.txt");
+    FILE *flp = open("linear_pressure.txt"), *fh = open("velocity_distribution.txt");
+    FILE *all[] = {fe, ft, fp, fq, fv, fk, fu, flt, flp, fh};
+    bool ok = true;
+    for (FILE *f : all) ok = ok && f;
+    if (ok) {
+        double T = 0, Q = 0;
+        for (int64_t i = 0; i < ns; i++) {
+            const double t = h->stats_t0 + (double)(steps[i] - h->stats_step0) * dt;
+            const double t_ns = t * t0u * 1e9;                                    // system.h:116
+            const double ke = v2[i] * (0.5 * mass * fn);                          // 21-32
+            std::fprintf(fe, "%f %E\n", t_ns, ke * E0ev);                       // 36
+            T = 2.0 / 3.0 * ke / (double)atoms;                                   // 62
+            std::fprintf(ft, "%f %f\n", t_ns, T * T0);                           // 64
+            std::fprintf(fp, "%f %E\n", t_ns, P0 * ((double)atoms / volume_global * T));   // 156-157
+            const double flux = fn * (double)per[i] / (t - h->stats_t0);         // 93
+            std::fprintf(fq, "%f %E\n", t_ns, flux);                            // 97
+            Q = flux * (1.0 / m.s.density);                                       // 124-125
+            std::fprintf(fv, "%f %E\n", t_ns, Q);                               // 127
+        }
+        // MeasurePermeability (180-216) from the last sample
+        const int fd = m.s.flow_direction;
+        double area = 1;
+        for (int a = 0; a < 3; a++)
+            if (a != fd) area *= L[a];
+        const double visc = 5.0 / (16 * m.s.diam * m.s.diam) * std::sqrt(mass * T / M_PI);
+        const double k = Q * visc / (area * mass * m.s.density * m.s.gravity);
+        std::fprintf(fk, "%E\n", k * perm0 * 1e4);
+        // the finalize of the binned properties (292-297, 343-351, 387-395, 433-441)
+        const double vpb = volume_global / (double)nb;
+        for (size_t i = 0; i < nb; i++) {
+            const double c = (double)std::max<uint64_t>(cnt[i], 1);
+            std::fprintf(fu, "%E ", v0 * (vz[i] / c));
+            const double Tb = ez[i] / c * (2.0 / 3.0);
+            std::fprintf(flt, "%E ", T0 * Tb);
+            std::fprintf(flp, "%E ", P0 * ((double)cnt[i] / vpb * Tb));
+        }
+        const double sd = std::sqrt(m.s.temperature / mass), vmin = -4 * sd, bin = (4 * sd - vmin) / (double)nb;
+        for (size_t i = 0; i < nb; i++)
+            std::fprintf(fh, "%f %ld\n", v0 * (vmin + (i + 0.5) * bin), (long)hist[i]);
+    }
+    for (FILE *f : all)
+        if (f) std::fclose(f);
+    if (!ok) return fail(h, DSMC_ERR_IO, "cannot write the statistics files");
     return DSMC_OK;
 }
 

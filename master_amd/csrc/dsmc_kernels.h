@@ -131,6 +131,32 @@ hipError_t launch_bucket(const BucketArgs &a, hipStream_t s);
 hipError_t launch_bucket_cells(const BucketArgs &a, hipStream_t s);
 hipError_t launch_gather(const GatherArgs &a, uint32_t ncells, hipStream_t s);
 // gather, NTC collisions and sampling in one pass (do_collide / do_sample)
+// The statistics sampler's per-sample reduction (StatisticsSampler,
+// statisticalproperty.cpp): particles binned along y, v_x histogram,
+// kinetic energy; the per-block partials are summed in block order.
+static constexpr uint32_t STATS_BLOCKS = 512;
+static constexpr uint32_t STATS_MAX_BINS = 1024;
+struct StatsArgs {
+    const float *oy, *vx, *vy, *vz;
+    const uint32_t *vox;
+    const uint32_t *step;
+    const unsigned long long *counters;
+    uint32_t n, interval, bins, flow_axis, series_cap;
+    double y_h;                 // voxel edge along y
+    double y_inv_len;           // one_over_length[1]
+    double v_min, v_bin;        // v_x histogram: lower edge, bin width
+    double half_m;              // 0.5 * mass
+    uint32_t *p_cnt, *p_vxh;    // per block partials [STATS_BLOCKS][bins]
+    double *p_vz, *p_ez, *p_ke; // (p_ke: [STATS_BLOCKS])
+    unsigned long long *cnt, *vxh;
+    double *vz_sum, *ez_sum;    // accumulated over samples [bins]
+    double *s_ke;               // per sample: sum of v^2
+    long long *s_per;           //   count_periodic on the flow axis
+    uint32_t *s_step;           //   step
+    unsigned long long *nsamples;
+};
+hipError_t launch_stats(const StatsArgs &a, hipStream_t s);
+
 hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t chunk, hipStream_t s);
 // chunk: cells per work-queue entry (0: automatic)
 hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, uint32_t chunk, hipStream_t s);

@@ -1444,6 +1444,94 @@ hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, uint32_t chunk_
     return hipSuccess;
 }
 
+// ------------------------------------------------------------------ statistics
+// One sample of MeasureCount, MeasureVelocityDistributionPoiseuille,
+// MeasureTemperatureDistribution (statisticalproperty.cpp:229-370: bin
+// int(r_y one_over_length_y bins), sums of 1, v_z and 0.5 m v_z v_z),
+// MeasureVelocityDistribution (409-430: bin int((v_x - v_min) / bin_size),
+// kept when in range) and MeasureEnergy (21-32: sum of v^2), taken when the
+// step count is a multiple of the interval (statisticalproperty.cpp:22).
+// Bins accumulate in LDS (FP64 sums by LDS atomics: their order, and so the
+// last bits, can vary from run to run), then one row per block.
+__global__ __launch_bounds__(256) void k_stats_bins(StatsArgs a) {
+    if (*a.step % a.interval) return;
+    __shared__ uint32_t lc[STATS_MAX_BINS], lh[STATS_MAX_BINS];
+    __shared__ double lvz[STATS_MAX_BINS], lez[STATS_MAX_BINS];
+    __shared__ double red[4];
+    const uint32_t bins = a.bins;
+    for (uint32_t b = threadIdx.x; b < bins; b += blockDim.x) {
+        lc[b] = 0; lh[b] = 0; lvz[b] = 0.0; lez[b] = 0.0;
+    }
+    __syncthreads();
+    double ke = 0.0;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += gridDim.x * blockDim.x) {
+        const uint32_t iy = (a.vox[p] >> 10) & 1023u;
+        const double r = ((double)iy + (double)a.oy[p]) * a.y_h;
+        int b = (int)(r * a.y_inv_len * (double)bins);
+        b = b < 0 ? 0 : (b >= (int)bins ? (int)bins - 1 : b);
+        const double x = a.vx[p], y = a.vy[p], z = a.vz[p];
+        atomicAdd(&lc[b], 1u);
+        atomicAdd(&lvz[b], z);
+        atomicAdd(&lez[b], a.half_m * z * z);
+        ke += x * x + y * y + z * z;
+        const int hb = (int)((x - a.v_min) / a.v_bin);
+        if (hb >= 0 && hb <= (int)bins - 1) atomicAdd(&lh[hb], 1u);
+    }
+    ke = wave_sum(ke);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ke;
+    __syncthreads();
+    const size_t row = (size_t)blockIdx.x * bins;
+    for (uint32_t b = threadIdx.x; b < bins; b += blockDim.x) {
+        a.p_cnt[row + b] = lc[b];
+        a.p_vxh[row + b] = lh[b];
+        a.p_vz[row + b] = lvz[b];
+        a.p_ez[row + b] = lez[b];
+    }
+    if (threadIdx.x == 0) a.p_ke[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// The block rows summed in block order into the accumulators; the sample's
+// energy, count_periodic and step appended to the series.
+__global__ __launch_bounds__(256) void k_stats_final(StatsArgs a) {
+    const uint32_t step = *a.step;
+    if (step % a.interval) return;
+    for (uint32_t b = threadIdx.x; b < a.bins; b += blockDim.x) {
+        unsigned long long c = 0, h = 0;
+        double vz = 0.0, ez = 0.0;
+        for (uint32_t k = 0; k < STATS_BLOCKS; k++) {
+            const size_t i = (size_t)k * a.bins + b;
+            c += a.p_cnt[i];
+            h += a.p_vxh[i];
+            vz += a.p_vz[i];
+            ez += a.p_ez[i];
+        }
+        a.cnt[b] += c;
+        a.vxh[b] += h;
+        a.vz_sum[b] += vz;
+        a.ez_sum[b] += ez;
+    }
+    if (threadIdx.x == 0) {
+        double ke = 0.0;
+        for (uint32_t k = 0; k < STATS_BLOCKS; k++) ke += a.p_ke[k];
+        const unsigned long long ns = *a.nsamples;
+        if (ns < a.series_cap) {
+            a.s_ke[ns] = ke;
+            a.s_step[ns] = step;
+            a.s_per[ns] = a.flow_axis < 3 ? (long long)a.counters[CNT_PER0 + a.flow_axis] : 0ll;
+        }
+        *a.nsamples = ns + 1;
+    }
+}
+
+hipError_t launch_stats(const StatsArgs &a, hipStream_t s) {
+    if (a.bins == 0 || a.bins > STATS_MAX_BINS || a.interval == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_stats_bins, dim3(STATS_BLOCKS), dim3(256), 0, s, a);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, s, a);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
 hipError_t launch_seed_fields(const SeedFieldsArgs &a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_seed_fields, dim3(2048), dim3(256), 0, s, a);

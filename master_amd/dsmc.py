@@ -81,7 +81,8 @@ def lib():
                      "dsmc_num_particles", "dsmc_settings_from_ini", "dsmc_world_read",
                      "dsmc_geometry_packed_spheres", "dsmc_geometry_poly_spheres", "dsmc_world_write",
                      "dsmc_save_state_file", "dsmc_load_state_file", "dsmc_initialize_fields",
-                     "dsmc_autotune"):
+                     "dsmc_autotune", "dsmc_set_statistics", "dsmc_get_statistics",
+                     "dsmc_get_statistics_series", "dsmc_write_statistics"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.dsmc_destroy.restype = None
         _lib.dsmc_world_free.restype = None
@@ -311,6 +312,30 @@ class System:
 
     def reset_moments(self):
         self._call("dsmc_reset_cell_moments")
+
+    # StatisticsSampler (statisticssampler.cpp:13-48)
+    def set_statistics(self, interval, bins=100, max_samples=100_000):
+        self._statistics_bins = int(bins)
+        self._call("dsmc_set_statistics", ctypes.c_int(interval), ctypes.c_int(bins), ctypes.c_int(max_samples))
+
+    def statistics(self):
+        nb = self._statistics_bins
+        ns = I64()
+        cnt, hist = np.zeros(nb, np.uint64), np.zeros(nb, np.uint64)
+        vz, ez = np.zeros(nb), np.zeros(nb)
+        self._call("dsmc_get_statistics", ctypes.byref(ns), _ptr(cnt), _ptr(vz), _ptr(ez), _ptr(hist))
+        return {"samples": ns.value, "count": cnt, "vz_sum": vz, "ez_sum": ez, "vx_hist": hist}
+
+    def statistics_series(self, cap=1_000_000):
+        n = I64(cap)
+        steps, per = np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+        v2 = np.zeros(cap)
+        self._call("dsmc_get_statistics_series", ctypes.byref(n), _ptr(steps), _ptr(v2), _ptr(per))
+        k = min(n.value, cap)
+        return {"steps": steps[:k], "v2_sum": v2[:k], "periodic": per[:k]}
+
+    def write_statistics(self, directory):
+        self._call("dsmc_write_statistics", os.fsencode(str(directory)))
 
     def totals(self):
         t = np.zeros(4)

@@ -376,6 +376,7 @@ class RefSystem:
         if workdir is None:
             self.tmp = tempfile.TemporaryDirectory(prefix="dsmcref_")
             workdir = self.tmp.name
+        self.workdir = workdir
         os.makedirs(os.path.join(workdir, "statistics", "time"), exist_ok=True)
         write_world_file(os.path.join(workdir, "world"), world)
         ini = dict(ini)
@@ -404,6 +405,19 @@ class RefSystem:
 
     def step(self):
         self.r.ref_sys_step(P(self.h))
+
+    def stats_enable(self):
+        self.r.ref_sys_stats_enable(P(self.h))
+
+    def stats_sample(self):
+        self.r.ref_sys_stats_sample(P(self.h))
+
+    def stats_finalize(self):
+        """Runs the reference's finalize and closes its statistics files;
+        returns their text by name."""
+        self.r.ref_sys_stats_finalize(P(self.h))
+        d = os.path.join(self.workdir, "statistics")
+        return {f: open(os.path.join(d, f)).read() for f in os.listdir(d) if f.endswith(".txt")}
 
     def counters(self):
         o = np.zeros(8)

@@ -14,6 +14,8 @@
 #include <random.h>
 #include <settings.h>
 #include <cvector.h>
+#include <statisticssampler.h>
+#include <dsmc_io.h>
 #include <unistd.h>
 #include <climits>
 #include <cstring>
@@ -45,6 +47,7 @@ struct Handle {
     Settings *settings;
     System *sys;
     RecordingCollider *rec;
+    StatisticsSampler *sampler = nullptr;
 };
 
 void ensure_mpi() {
@@ -99,6 +102,23 @@ void ref_sys_move_all(void *hp, double dt) {
 }
 
 void ref_sys_step(void *hp) { ((Handle *)hp)->sys->step(); }
+
+// The reference's StatisticsSampler (statisticssampler.cpp:13-48) on the
+// statistics/ files System::initialize opened in the work directory
+// (dsmc_io.cpp:33-46): sample() after each step as main.cpp:58-64 does,
+// finalize() and the files' close (dsmc_io.cpp:160-182) at the end.
+void ref_sys_stats_enable(void *hp) {
+    Handle *h = (Handle *)hp;
+    if (!h->sampler) h->sampler = new StatisticsSampler(h->sys);
+}
+
+void ref_sys_stats_sample(void *hp) { ((Handle *)hp)->sampler->sample(); }
+
+void ref_sys_stats_finalize(void *hp) {
+    Handle *h = (Handle *)hp;
+    h->sampler->finalize();
+    h->sys->io->finalize();
+}
 
 // out: steps, t, collisions, wall collisions, count_periodic[3], log length
 void ref_sys_counters(void *hp, double *out) {

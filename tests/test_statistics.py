@@ -1,0 +1,120 @@
+"""The statistics sampler (statisticssampler.cpp:13-48, statisticalproperty.cpp)
+and its files: the CPU restatement (oracle/stats_oracle.py) against the
+compiled reference's own sampler on the reference's states, then the device
+sampler and the library's writer against the restatement on the device's
+states."""
+import numpy as np
+import pytest
+
+from master_amd import config as C
+from oracle import binding as B
+from oracle import stats_oracle as S
+from tests import dsmc_cases as K
+
+INTERVAL, BINS, STEPS = 2, 20, 8
+FILES = ("energy.txt", "temperature.txt", "pressure.txt", "number_flow_rate.txt", "volumetric_flow_rate.txt",
+         "permeability.txt", "velocity.txt", "linear_temperature.txt", "linear_pressure.txt",
+         "velocity_distribution.txt")
+
+
+def _ini(w, n=20000):
+    ini = K.settings(w, n, gravity=0.5)
+    ini["statistics_interval"], ini["sampling_bins"] = INTERVAL, BINS
+    return ini
+
+
+def _params(ini, w, n, t0=0.0):
+    u = B.units()
+    return {"mass": float(ini["mass"]), "temperature": float(ini["temperature"]) / u["T0"],
+            "fn": int(float(ini["atoms_per_molecule"])), "n": n,
+            "L": [float(ini["Lx"]), float(ini["Ly"]), float(ini["Lz"])], "porosity": float(w["porosity"]),
+            "density": float(ini["density"]) / u["number_density0"], "diam": float(ini["diam"]),
+            "gravity": float(ini["gravity"]), "flow_direction": int(ini["flow_direction"]), "t0": t0}
+
+
+def _restate(states, times, periodic, p):
+    sums, series = None, []
+    for st, t, per in zip(states, times, periodic):
+        s = S.sample(st, p["mass"], p["temperature"], p["L"], BINS)
+        series.append({"t": t, "v2_sum": s["v2_sum"], "periodic": per})
+        if sums is None:
+            sums = {k: s[k].copy() for k in ("count", "vz_sum", "ez_sum", "vx_hist")}
+        else:
+            for k in sums:
+                sums[k] = sums[k] + s[k]
+    return S.files(series, sums, p), sums, series
+
+
+def _same_numbers(a, b, name):
+    x, y = S.numbers(a), S.numbers(b)
+    assert x.shape == y.shape, name
+    # printf's 7 significant digits (%E) or 6 decimals (%f, the times)
+    assert np.allclose(x, y, rtol=3e-6, atol=2e-6), (name, np.max(np.abs(x - y) / np.maximum(np.abs(y), 1e-300)))
+
+
+@pytest.mark.skipif(not B.ref_available(), reason="reference harness not built")
+def test_restatement_reproduces_the_reference_sampler():
+    w = K.world()
+    ini = _ini(w)
+    r = B.RefSystem(ini, w)
+    r.stats_enable()
+    states, times, periodic = [], [], []
+    for _ in range(STEPS):
+        r.step()
+        r.stats_sample()
+        c = r.counters()
+        if c["steps"] % INTERVAL == 0:
+            states.append(r.state())
+            times.append(c["t"])
+            periodic.append(c["count_periodic"][int(ini["flow_direction"])])
+    ref = r.stats_finalize()
+    mine, _, _ = _restate(states, times, periodic, _params(ini, w, r.n))
+    for f in FILES:
+        _same_numbers(mine[f], ref[f], f)
+    assert S.numbers(mine["number_flow_rate.txt"])[1::2].any()     # particles crossed the flow face
+
+
+@pytest.mark.gpu
+def test_device_sampler_and_writer_match_restatement(gpu, tmp_path):
+    """Bins counted exactly, FP64 sums to rounding; the written files equal
+    the restatement's on the device's own states."""
+    dsmc = gpu
+    w = K.world()
+    ini = _ini(w, 100_000)
+    g = dsmc.System(ini, w)
+    g.initialize()
+    g.set_statistics(INTERVAL, BINS, 64)
+    states, times, periodic = [], [], []
+    for k in range(1, STEPS + 1):
+        g.step(1)
+        if k % INTERVAL == 0:
+            c = g.counters()
+            states.append(g.state())
+            times.append(c["t"])
+            periodic.append(c["count_periodic"][int(ini["flow_direction"])])
+    dev = g.statistics()
+    ser = g.statistics_series()
+    mine, sums, series = _restate(states, times, periodic, _params(ini, w, g.n))
+    assert dev["samples"] == len(states) == len(ser["steps"])
+    assert np.array_equal(ser["steps"], np.arange(1, len(states) + 1) * INTERVAL)
+    assert np.array_equal(ser["periodic"], periodic)
+    assert np.allclose(ser["v2_sum"], [s["v2_sum"] for s in series], rtol=1e-12)
+    assert np.array_equal(dev["count"], sums["count"])
+    assert np.array_equal(dev["vx_hist"], sums["vx_hist"])
+    assert np.allclose(dev["vz_sum"], sums["vz_sum"], rtol=1e-9, atol=1e-12)
+    assert np.allclose(dev["ez_sum"], sums["ez_sum"], rtol=1e-9)
+    g.write_statistics(tmp_path)
+    for f in FILES:
+        _same_numbers(open(tmp_path / f).read(), mine[f], f)
+    # graph replay (several steps per launch) samples the same steps
+    g2 = dsmc.System(ini, w)
+    g2.initialize()
+    g2.set_option(dsmc.OPT_USE_GRAPH, 1)
+    g2.set_statistics(INTERVAL, BINS, 64)
+    g2.step(STEPS)
+    d2 = g2.statistics()
+    assert d2["samples"] == dev["samples"]
+    assert np.array_equal(d2["count"], dev["count"]) and np.array_equal(d2["vx_hist"], dev["vx_hist"])
+    assert np.allclose(d2["vz_sum"], dev["vz_sum"], rtol=1e-12, atol=1e-14)
+    g.close()
+    g2.close()
