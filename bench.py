@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--lds-cap", type=int, default=0, help="particles per cell held in LDS (0: automatic)")
+    ap.add_argument("--statistics", type=int, default=0,
+                    help="also run the statistics sampler every N steps (100 y bins); off by default")
     ap.add_argument("--pmc-pass", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-steps", type=int, default=2, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-timeout", type=int, default=150, help=argparse.SUPPRESS)
@@ -293,6 +295,8 @@ def main():
     if args.workload == "ensemble":
         return bench_ensemble(args)
     p, w, g, n = make_system(args)
+    if args.statistics:
+        g.set_statistics(args.statistics, 100, 1_000_000)
 
     # per-kernel device times (HIP events on the handle's stream), eager launches
     g.set_option(dsmc.OPT_TIMING, 1)
@@ -300,6 +304,10 @@ def main():
     ktimes = g.kernel_times()
     g.set_option(dsmc.OPT_TIMING, 0)
     log(f"[bench] kernel ms/launch: {ktimes}")
+    if args.statistics and ktimes.get("k_stats"):
+        # a sample reads oy, vox, vx, vy, vz: 20 B per particle
+        log(f"[bench] statistics sample: {ktimes['k_stats']:.3f} ms, "
+            f"{20 * n / (ktimes['k_stats'] * 1e-3) / 1e9:.0f} GB/s of particle state read")
 
     g.set_option(dsmc.OPT_USE_GRAPH, 1)
     tune = None

@@ -1490,32 +1490,49 @@ __global__ __launch_bounds__(256) void k_stats_bins(StatsArgs a) {
     if (threadIdx.x == 0) a.p_ke[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// The block rows summed in block order into the accumulators; the sample's
-// energy, count_periodic and step appended to the series.
+// The block rows summed in a fixed order into the accumulators (a block of
+// four waves per 64 bins, each wave a quarter of the rows, the quarters
+// added in order); block 0 also appends the sample's energy, count_periodic
+// and step to the series.
 __global__ __launch_bounds__(256) void k_stats_final(StatsArgs a) {
     const uint32_t step = *a.step;
     if (step % a.interval) return;
-    for (uint32_t b = threadIdx.x; b < a.bins; b += blockDim.x) {
-        unsigned long long c = 0, h = 0;
-        double vz = 0.0, ez = 0.0;
-        for (uint32_t k = 0; k < STATS_BLOCKS; k++) {
+    __shared__ unsigned long long sc[4][64], sh[4][64];
+    __shared__ double svz[4][64], sez[4][64], ske[4];
+    const uint32_t lane = threadIdx.x & 63u, q = threadIdx.x >> 6;
+    const uint32_t b = blockIdx.x * 64 + lane;
+    constexpr uint32_t ROWS = STATS_BLOCKS / 4;
+    unsigned long long c = 0, h = 0;
+    double vz = 0.0, ez = 0.0;
+    if (b < a.bins) {
+#pragma unroll 8
+        for (uint32_t k = q * ROWS; k < (q + 1) * ROWS; k++) {
             const size_t i = (size_t)k * a.bins + b;
             c += a.p_cnt[i];
             h += a.p_vxh[i];
             vz += a.p_vz[i];
             ez += a.p_ez[i];
         }
-        a.cnt[b] += c;
-        a.vxh[b] += h;
-        a.vz_sum[b] += vz;
-        a.ez_sum[b] += ez;
     }
-    if (threadIdx.x == 0) {
-        double ke = 0.0;
-        for (uint32_t k = 0; k < STATS_BLOCKS; k++) ke += a.p_ke[k];
+    sc[q][lane] = c; sh[q][lane] = h; svz[q][lane] = vz; sez[q][lane] = ez;
+    double ke = 0.0;
+    if (blockIdx.x == 0) {
+        for (uint32_t k = threadIdx.x; k < STATS_BLOCKS; k += 256) ke += a.p_ke[k];
+        ke = wave_sum(ke);
+        if (lane == 0) ske[q] = ke;
+    }
+    __syncthreads();
+    if (q == 0 && b < a.bins) {
+        a.cnt[b] += ((sc[0][lane] + sc[1][lane]) + sc[2][lane]) + sc[3][lane];
+        a.vxh[b] += ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
+        a.vz_sum[b] += ((svz[0][lane] + svz[1][lane]) + svz[2][lane]) + svz[3][lane];
+        a.ez_sum[b] += ((sez[0][lane] + sez[1][lane]) + sez[2][lane]) + sez[3][lane];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double total = ((ske[0] + ske[1]) + ske[2]) + ske[3];
         const unsigned long long ns = *a.nsamples;
         if (ns < a.series_cap) {
-            a.s_ke[ns] = ke;
+            a.s_ke[ns] = total;
             a.s_step[ns] = step;
             a.s_per[ns] = a.flow_axis < 3 ? (long long)a.counters[CNT_PER0 + a.flow_axis] : 0ll;
         }
@@ -1527,7 +1544,7 @@ hipError_t launch_stats(const StatsArgs &a, hipStream_t s) {
     if (a.bins == 0 || a.bins > STATS_MAX_BINS || a.interval == 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_stats_bins, dim3(STATS_BLOCKS), dim3(256), 0, s, a);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_stats_final, dim3((a.bins + 63) / 64), dim3(256), 0, s, a);
     LAUNCH_CHECK();
     return hipSuccess;
 }
