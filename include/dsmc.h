@@ -219,11 +219,15 @@ enum {
     DSMC_OPT_COLLIDE_CHUNK = 7, /* cells per collide work-queue entry, 1..64 (0: automatic) */
     DSMC_OPT_GRAPH_STEPS = 8,   /* steps per captured graph, even, 2..64 (default 2) */
     DSMC_OPT_FUSED = 9,         /* 1 (default): gather, collide and sampling in one pass */
-    DSMC_OPT_MOVE_QUEUE = 10    /* wall-path particles the mover queues per cell, 0..512 (default
+    DSMC_OPT_MOVE_QUEUE = 10,   /* wall-path particles the mover queues per cell, 0..512 (default
                                  * 512); the rest take the overflow pass */
+    DSMC_OPT_MOVE_AHEAD = 11    /* 1 (default): inside one dsmc_step call the fused pass of a step
+                                 * also moves for the next step (System::step's move follows its
+                                 * collide); off with recording, totals, statistics, partial
+                                 * phases or Cercignani-Lampis walls */
 };
 typedef struct {
-    int graph_steps;            /* chosen steps per captured graph */
+    int graph_steps;            /* chosen steps per captured graph (2..32) */
     int collide_chunk;          /* chosen cells per collide queue entry (0: automatic) */
     double ms_per_step;         /* best measured configuration */
     double ms_per_step_default; /* the default configuration, same measurement */

@@ -76,6 +76,8 @@ struct dsmc_handle {
     bool lds_auto = true;
     uint32_t collide_chunk = 0;     // 0: automatic
     uint32_t move_queue = MOVE_SLOWQ;  // mover's per-cell wall-path queue in use
+    bool move_ahead = true;         // the fused pass also moves for the next step of a dsmc_step call
+    uint32_t *d_slow_idx = nullptr, *d_slow_cnt = nullptr;   // move-ahead wall-path lists
     bool fused = true;              // gather and collide in one pass
     int graph_steps = 2;            // steps per captured graph (even)
     // host mirrors of System counters
@@ -287,12 +289,13 @@ dsmc_status upload_geometry(dsmc_handle *h) {
     HIPCHK(h, dalloc(&h->d_start[0], cells + 1));
     HIPCHK(h, dalloc(&h->d_start[1], cells + 1));
     HIPCHK(h, dalloc(&h->d_leave, cells));
+    HIPCHK(h, dalloc(&h->d_slow_cnt, cells));
     HIPCHK(h, dalloc(&h->d_arrive, cells));
     HIPCHK(h, dalloc(&h->d_bstart, cells + 1));
     HIPCHK(h, dalloc(&h->d_bfill, cells));
     HIPCHK(h, dalloc(&h->d_bsums, h->nblocks));
     HIPCHK(h, dalloc(&h->d_step, 1));
-    HIPCHK(h, dalloc(&h->d_mvcount, 3));
+    HIPCHK(h, dalloc(&h->d_mvcount, 4));
     HIPCHK(h, dalloc(&h->d_cnt, CNT_NUM));
     HIPCHK(h, dalloc(&h->d_partial, 2 * TOT_BLOCKS * 4));
     HIPCHK(h, hipMemcpy(h->d_members, members.data(), members.size() * sizeof(Member), hipMemcpyHostToDevice));
@@ -318,7 +321,7 @@ void free_particles(dsmc_handle *h) {
         dfree(h->vx[b]); dfree(h->vy[b]); dfree(h->vz[b]);
         dfree(h->vox[b]);
     }
-    dfree(h->d_mv_idx); dfree(h->d_mv_cell); dfree(h->d_bucket);
+    dfree(h->d_mv_idx); dfree(h->d_mv_cell); dfree(h->d_bucket); dfree(h->d_slow_idx);
     dfree(h->d_rec_perm); dfree(h->d_rec_nhit); dfree(h->d_rec_last);
 }
 
@@ -345,6 +348,7 @@ dsmc_status alloc_particles(dsmc_handle *h, int64_t n) {
     HIPCHK(h, dalloc(&h->d_mv_idx, n));
     HIPCHK(h, dalloc(&h->d_mv_cell, n));
     HIPCHK(h, dalloc(&h->d_bucket, n));
+    HIPCHK(h, dalloc(&h->d_slow_idx, n));
     if (h->record) {
         HIPCHK(h, dalloc(&h->d_rec_perm, n));
         HIPCHK(h, dalloc(&h->d_rec_nhit, n));
@@ -437,6 +441,15 @@ GatherArgs gather_args(dsmc_handle *h, bool collide, bool sample) {
     g.lds_cap = h->lds_cap;
     g.do_collide = collide ? 1 : 0;
     g.do_sample = sample ? 1 : 0;
+    // the next step's mover lists (move-ahead launches)
+    g.types = h->d_types;
+    g.frames = h->d_frames;
+    g.leave = h->d_leave;
+    g.arrive = h->d_arrive;
+    g.mv_idx = h->d_mv_idx;
+    g.mv_cell = h->d_mv_cell;
+    g.slow_idx = h->d_slow_idx;
+    g.slow_count = h->d_slow_cnt;
     return g;
 }
 
@@ -565,11 +578,51 @@ void free_stats(dsmc_handle *h) {
     h->stats_interval = 0;
 }
 
-dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
+// Whether the fused pass may move for the next step: every phase on, one
+// fused pass, nothing that reads the state between steps.
+bool move_ahead_ok(const dsmc_handle *h) {
+    if (!h->move_ahead || !h->fused || h->phases != 7 || h->record || h->totals || h->stats_interval) return false;
+    for (auto &m : h->mem)
+        if (m.dev.wall_model == DSMC_WALL_CERCIGNANI_LAMPIS) return false;
+    return true;
+}
+
+// MoveArgs of the move-ahead's wall path (k_move_slow): the buffer and cell
+// ranges the pass has just produced, the next step's draws.
+MoveArgs slow_args(dsmc_handle *h) {
+    MoveArgs m;
+    std::memset(&m, 0, sizeof m);
+    int a = h->cur;
+    m.ox = h->ox[a]; m.oy = h->oy[a]; m.oz = h->oz[a]; m.vx = h->vx[a]; m.vy = h->vy[a]; m.vz = h->vz[a];
+    m.vox = h->vox[a];
+    m.start = h->d_start[h->scur];
+    m.leave = h->d_leave;
+    m.arrive = h->d_arrive;
+    m.mv_idx = h->d_mv_idx;
+    m.mv_cell = h->d_mv_cell;
+    m.members = h->d_members;
+    m.cell_member = h->d_cell_member;
+    m.types = h->d_types;
+    m.frames = h->d_frames;
+    m.step = h->d_step;
+    m.counters = h->d_cnt;
+    m.ncells = h->ncells;
+    m.step_offset = 1;
+    m.slow_idx = h->d_slow_idx;
+    m.slow_count = h->d_slow_cnt;
+    return m;
+}
+
+// One step.  move_prev: the previous step's fused pass has already moved
+// this step (no k_move); move_next: this step's fused pass moves for the
+// next one (System::step's move of step k+1 follows its collide of step k).
+dsmc_status enqueue_step(dsmc_handle *h, bool sample, bool move_prev = false, bool move_next = false) {
     hipStream_t s = h->stream;
     HIPCHK(h, launch_begin_step(h->d_step, h->d_mvcount, s));
-    HIPCHK(h, launch_fill_u32(h->d_arrive, 0u, h->ncells, s));
-    if (h->phases & 1) {
+    if (!move_prev) HIPCHK(h, launch_fill_u32(h->d_arrive, 0u, h->ncells, s));
+    if (move_prev) {
+        // moved by the previous step's pass
+    } else if (h->phases & 1) {
         MoveArgs m;
         int a = h->cur;
         m.ox = h->ox[a]; m.oy = h->oy[a]; m.oz = h->oz[a]; m.vx = h->vx[a]; m.vy = h->vy[a]; m.vz = h->vz[a];
@@ -597,6 +650,9 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
         for (auto &mb : h->mem) m.any_cl |= mb.dev.wall_model == DSMC_WALL_CERCIGNANI_LAMPIS;
         m.cell_begin = 0;
         m.ncells = h->ncells;
+        m.step_offset = 0;
+        m.slow_idx = nullptr;
+        m.slow_count = nullptr;
         Scope sc(h, "k_move");
         HIPCHK(h, launch_move(m, h->ncells, s));
     } else {
@@ -616,10 +672,19 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample) {
     }
     const bool collide = (h->phases & 4) != 0;
     if (h->fused && (collide || sample)) {
-        Scope sc(h, "k_gather_collide");
-        HIPCHK(h, launch_gather_collide(gather_args(h, collide, sample), h->ncells, h->collide_chunk, s));
+        // the next step's arrivals are counted by this pass (this step's
+        // were read by the scan)
+        if (move_next) HIPCHK(h, launch_fill_u32(h->d_arrive, 0u, h->ncells, s));
+        {
+            Scope sc(h, "k_gather_collide");
+            HIPCHK(h, launch_gather_collide(gather_args(h, collide, sample), h->ncells, h->collide_chunk, move_next, s));
+        }
         h->cur = 1 - h->cur;
         h->scur = 1 - h->scur;
+        if (move_next) {
+            Scope sc(h, "k_move_slow");
+            HIPCHK(h, launch_move_slow(slow_args(h), h->ncells, s));
+        }
     } else {
         {
             Scope sc(h, "k_gather");
@@ -652,7 +717,9 @@ dsmc_status capture_graph(dsmc_handle *h, int parity, bool sample) {
     h->timing = false;
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     dsmc_status st = DSMC_OK;
-    for (int k = 0; k < h->graph_steps && st == DSMC_OK; k++) st = enqueue_step(h, sample);
+    const bool ahead = move_ahead_ok(h);
+    for (int k = 0; k < h->graph_steps && st == DSMC_OK; k++)
+        st = enqueue_step(h, sample, ahead && k > 0, ahead && k < h->graph_steps - 1);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     h->timing = save_timing;
@@ -773,6 +840,7 @@ void dsmc_destroy(dsmc_handle *h) {
     dfree(h->d_pairs); dfree(h->d_acc);
     dfree(h->d_start[0]); dfree(h->d_start[1]);
     dfree(h->d_leave); dfree(h->d_arrive); dfree(h->d_bstart); dfree(h->d_bfill); dfree(h->d_bsums);
+    dfree(h->d_slow_cnt);
     dfree(h->d_step); dfree(h->d_mvcount); dfree(h->d_cnt); dfree(h->d_partial);
     free_stats(h);
     for (auto &t : h->timers)
@@ -1016,6 +1084,11 @@ dsmc_status dsmc_step(dsmc_handle *h, int nsteps) {
     if (h->timing)
         for (auto &t : h->timers) t.used = 0;
     int done = 0;
+    // eager steps chain the next step's move into the fused pass; a chain
+    // (and every captured graph) starts with k_move and ends without the
+    // next move, so the state between dsmc_step calls is a whole step's
+    const bool ahead = move_ahead_ok(h);
+    bool moved = false;
     while (done < nsteps) {
         bool sample = h->sample_every > 0 && ((h->steps + 1) % h->sample_every) == 0;
         bool graph_ok = h->use_graph && !h->timing && nsteps - done >= h->graph_steps &&
@@ -1041,8 +1114,10 @@ dsmc_status dsmc_step(dsmc_handle *h, int nsteps) {
             done += h->graph_steps;
             continue;
         }
-        dsmc_status st = enqueue_step(h, sample);
+        const bool next = ahead && done + 1 < nsteps;
+        dsmc_status st = enqueue_step(h, sample, moved, next);
         if (st != DSMC_OK) return st;
+        moved = next;
         h->steps++;
         h->t += h->mem[0].s.dt;
         if (sample) h->nsamples++;
@@ -1394,6 +1469,10 @@ dsmc_status dsmc_set_option(dsmc_handle *h, int option, int64_t value) {
         h->fused = value != 0;
         drop_graphs(h);
         return DSMC_OK;
+    case DSMC_OPT_MOVE_AHEAD:
+        h->move_ahead = value != 0;
+        drop_graphs(h);
+        return DSMC_OK;
     case DSMC_OPT_MOVE_QUEUE:
         if (value < 0 || value > (int64_t)MOVE_SLOWQ) return DSMC_ERR_INVALID_ARGUMENT;
         h->move_queue = (uint32_t)value;
@@ -1439,7 +1518,7 @@ dsmc_status dsmc_autotune(dsmc_handle *h, int steps_per_trial, dsmc_tuning *out)
     HIPCHK(h, hipEventCreate(&e1));
     struct Cand { int gs; uint32_t chunk; };
     std::vector<Cand> cands;
-    const int gsteps[] = {2, 4, 8};
+    const int gsteps[] = {2, 4, 8, 16, 32};
     const uint32_t chunks[] = {0, 1, 2, 4, 8, 16};
     for (int gs : gsteps)
         for (uint32_t ch : chunks) cands.push_back({gs, ch});

@@ -35,6 +35,8 @@ struct MoveArgs {
     uint32_t cell_begin, ncells;
     uint32_t slowq_cap;                 // queue entries in use, <= MOVE_SLOWQ
     int any_cl;                         // some member uses the Cercignani-Lampis wall model
+    uint32_t step_offset;               // 1: the next step's move (after a move-ahead pass)
+    const uint32_t *slow_idx, *slow_count;  // k_move_slow's lists (from the move-ahead pass)
 };
 
 struct ClassifyArgs {
@@ -78,6 +80,12 @@ struct GatherArgs {
     uint32_t ncells, cell_begin, lds_cap;
     uint32_t chunk;                     // cells per queue entry (set by launch_gather_collide)
     int do_collide, do_sample;
+    // the next step's move inside the fused pass (move-ahead launches only):
+    // the mover lists of the next step, indexed by the new cell ranges
+    const uint8_t *types;
+    const float *frames;
+    uint32_t *leave, *arrive, *mv_idx, *mv_cell;
+    uint32_t *slow_idx, *slow_count;    // wall-path particles per cell (new ranges), for k_move_slow
 };
 
 struct CollideArgs {
@@ -157,7 +165,11 @@ struct StatsArgs {
 };
 hipError_t launch_stats(const StatsArgs &a, hipStream_t s);
 
-hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t chunk, hipStream_t s);
+// move_ahead: the pass also moves the cells' particles for the next step
+// (then launch_move_slow with step_offset 1 finishes that move).
+hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t chunk, bool move_ahead,
+                                 hipStream_t s);
+hipError_t launch_move_slow(const MoveArgs &a, uint32_t ncells, hipStream_t s);
 // chunk: cells per work-queue entry (0: automatic)
 hipError_t launch_collide(const CollideArgs &a, uint32_t ncells, uint32_t chunk, hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);

@@ -234,11 +234,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DSMC_MOVE_WA
 
 // Particles whose cell's queue was full (k_move), one per lane: the cell
 // from the start array, the move as in k_move's deferred loop, the mover
-// slot from an atomic on the cell's leaver count (k_move has written it).
+// slot from an atomic on the cell's leaver count (k_move, or the move-ahead
+// pass, has written it).
 template <bool CL>
 __global__ __launch_bounds__(64) void k_move_overflow(MoveArgs a) {
     const uint32_t n = *a.n_overflow;
-    const uint32_t step = *a.step;
+    const uint32_t step = *a.step + a.step_offset;
     const auto kargs = (const __attribute__((address_space(4))) MoveArgs *)__builtin_amdgcn_kernarg_segment_ptr();
     // an empty tile: every voxel lookup goes to the member's voxel array
     VoxelView V;
@@ -285,6 +286,87 @@ __global__ __launch_bounds__(64) void k_move_overflow(MoveArgs a) {
         if (cnt.periodic[0]) atomicAdd(&a.counters[CNT_PER0], (unsigned long long)(long long)cnt.periodic[0]);
         if (cnt.periodic[1]) atomicAdd(&a.counters[CNT_PER1], (unsigned long long)(long long)cnt.periodic[1]);
         if (cnt.periodic[2]) atomicAdd(&a.counters[CNT_PER2], (unsigned long long)(long long)cnt.periodic[2]);
+    }
+}
+
+// The wall-path particles of a move-ahead pass (gc_write_back_move).  A
+// block of 64 lanes takes 64 consecutive cells, lays their listed particles
+// out as one queue (prefix sums of the cells' counts) and moves them 64 at a
+// time with move_general from their post-collision state, exactly as
+// k_move's deferred loop would; leavers are appended to their cell's mover
+// list behind the fast path's (slots from an LDS counter per cell: the
+// order of a mover list is free, the gather ranks arrivals by index).
+__global__ __launch_bounds__(64) void k_move_slow(MoveArgs a) {
+    __shared__ uint32_t qstart[WAVE + 1], lslot[WAVE];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t c0 = blockIdx.x * WAVE;
+    const uint32_t cl = c0 + lane;
+    const uint32_t cnt_c = cl < a.ncells ? a.slow_count[cl] : 0u;
+    const uint64_t any = __ballot(cnt_c != 0);
+    if (!any) return;
+    // exclusive prefix of the counts over the block's cells
+    uint32_t incl = cnt_c;
+#pragma unroll
+    for (uint32_t d = 1; d < WAVE; d <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)incl, d, WAVE);
+        if (lane >= d) incl += v;
+    }
+    qstart[lane] = incl - cnt_c;
+    if (lane == WAVE - 1) qstart[WAVE] = incl;
+    lslot[lane] = cl < a.ncells ? a.leave[cl] : 0u;
+    const uint32_t total = __builtin_amdgcn_readlane(incl, WAVE - 1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    const uint32_t step = *a.step + a.step_offset;
+    const auto kargs = (const __attribute__((address_space(4))) MoveArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    MoveCounters cnt = {0, 0, 0, {0, 0, 0}};
+    for (uint32_t q0 = 0; q0 < total; q0 += WAVE) {
+        const uint32_t j = q0 + lane;
+        if (j < total) {
+            // the cell holding queue entry j: the last k with qstart[k] <= j
+            uint32_t k = 0;
+            for (uint32_t w = WAVE / 2; w >= 1; w >>= 1)
+                if (qstart[k + w] <= j) k += w;
+            const uint32_t c = c0 + k;
+            const uint32_t mi = a.cell_member[c];
+            const Member M = a.members[mi];
+            VoxelView V;    // an empty tile: voxel lookups go to the member's array
+            V.tile = nullptr;
+            V.tx0 = V.ty0 = V.tz0 = 0;
+            V.TX = V.TY = V.TZ = 0;
+            V.n0 = M.n[0]; V.n1 = M.n[1]; V.n2 = M.n[2];
+            const uint32_t s0 = a.start[c];
+            const uint32_t q = at_off(a.slow_idx, (s0 + (j - qstart[k])) << 2);
+            const uint32_t qo = q << 2;
+            float qox = at_off(a.ox, qo), qoy = at_off(a.oy, qo), qoz = at_off(a.oz, qo);
+            float qvx = at_off(a.vx, qo), qvy = at_off(a.vy, qo), qvz = at_off(a.vz, qo);
+            int ix, iy, iz;
+            unpack_vox(at_off(a.vox, qo), ix, iy, iz);
+            uint32_t nhit = 0;
+            (void)move_general<false>(kargs, mi, V, step, q - M.pfirst, ix, iy, iz, qox, qoy, qoz, qvx, qvy, qvz,
+                                      cnt, nhit);
+            at_off(a.ox, qo) = qox; at_off(a.oy, qo) = qoy; at_off(a.oz, qo) = qoz;
+            at_off(a.vx, qo) = qvx; at_off(a.vy, qo) = qvy; at_off(a.vz, qo) = qvz;
+            at_off(a.vox, qo) = pack_vox(ix, iy, iz);
+            const uint32_t newc = cell_of(M, ix, iy, iz);
+            if (newc != c) {
+                const uint32_t so = (s0 + atomicAdd(&lslot[k], 1u)) << 2;
+                at_off(a.mv_idx, so) = q;
+                at_off(a.mv_cell, so) = newc;
+                atomicAdd(&a.arrive[newc], 1u);
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    if (cnt_c) a.leave[cl] = lslot[lane];
+    uint32_t wh = wave_sum(cnt.wall_hits), rs = wave_sum(cnt.reseeds), er = wave_sum(cnt.errors);
+    int32_t p0 = wave_sum(cnt.periodic[0]), p1 = wave_sum(cnt.periodic[1]), p2 = wave_sum(cnt.periodic[2]);
+    if (lane == 0) {
+        if (wh) atomicAdd(&a.counters[CNT_WALL], (unsigned long long)wh);
+        if (rs) atomicAdd(&a.counters[CNT_RESEED], (unsigned long long)rs);
+        if (er) atomicAdd(&a.counters[CNT_ERROR], (unsigned long long)er);
+        if (p0) atomicAdd(&a.counters[CNT_PER0], (unsigned long long)(long long)p0);
+        if (p1) atomicAdd(&a.counters[CNT_PER1], (unsigned long long)(long long)p1);
+        if (p2) atomicAdd(&a.counters[CNT_PER2], (unsigned long long)(long long)p2);
     }
 }
 
@@ -433,6 +515,8 @@ static constexpr int GATHER_BATCH = 8;      // chunks of 64 particles per round 
 // three waves per SIMD anyway (see launch_gather_collide).  The next cell's
 // batch is in registers during a cell's write-back.
 static constexpr int GC_BATCH_SMALL = 5, GC_BATCH_LARGE = 11;
+// with the next step's move in the pass (gc_write_back_move): 112 and 165 VGPRs
+static constexpr int GC_BATCH_LARGE_MOVE = 8;
 
 __global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
     const uint32_t c = blockIdx.x + a.cell_begin;
@@ -890,7 +974,9 @@ __device__ __forceinline__ void gc_load_batch(const GatherArgs &a, uint32_t base
 #pragma unroll
     for (int q = 0; q < GC_BATCH; q++) {
         const uint32_t p = base + q * WAVE + lane;
+        // every field written on every lane: no earlier value stays live
         P.vp[q] = 0xffffffffu;
+        P.ox[q] = P.oy[q] = P.oz[q] = P.vx[q] = P.vy[q] = P.vz[q] = 0.0f;
         if (p < oe) {
             P.vp[q] = a.vox[p];
             P.ox[q] = a.ox[p]; P.oy[q] = a.oy[p]; P.oz[q] = a.oz[p];
@@ -946,25 +1032,24 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
         n_low = wave_sum(n_low);
         if (keys_lds) __syncthreads();
     }
+    // at most 64 arrivals: rank among them (keys are distinct), their data
+    // loaded now and stored after the stayers, whose processing hides the
+    // loads' latency
+    const bool arrival = few && lane < nb;
+    float apx = 0.0f, apy = 0.0f, apz = 0.0f, avx = 0.0f, avy = 0.0f, avz = 0.0f;
+    uint32_t apv = 0, aL = 0;
     if (few && nb > 0) {
-        // rank among the arrivals (keys are distinct), then the data
         uint32_t rank = 0;
         for (uint32_t j = 0; j < nb; j++) {
             const uint32_t kj = (uint32_t)__shfl((int)key0, (int)j, WAVE);
             rank += kj < key0 ? 1u : 0u;
         }
-        if (lane < nb) {
+        if (arrival) {
             const uint32_t q = key0;
-            const float px = a.ox[q], py = a.oy[q], pz = a.oz[q];
-            const uint32_t pv = a.vox[q];
-            const float x = a.vx[q], y = a.vy[q], z = a.vz[q];
-            const uint32_t L = rank < n_low ? rank : n_stay + rank;
-            const uint32_t d = ns + L;
-            a.ox2[d] = px; a.oy2[d] = py; a.oz2[d] = pz;
-            a.vox2[d] = pv;
-            if (in_lds) { lx[L] = x; ly[L] = y; lz[L] = z; }
-            else { a.vx2[d] = x; a.vy2[d] = y; a.vz2[d] = z; }
-            if (a.rec_perm) a.rec_perm[d] = (int32_t)q;
+            apx = a.ox[q]; apy = a.oy[q]; apz = a.oz[q];
+            apv = a.vox[q];
+            avx = a.vx[q]; avy = a.vy[q]; avz = a.vz[q];
+            aL = rank < n_low ? rank : n_stay + rank;
         }
     }
     uint32_t run = 0;
@@ -991,6 +1076,14 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
             }
             run += (uint32_t)__popcll(sm);
         }
+    }
+    if (arrival) {
+        const uint32_t d = ns + aL;
+        a.ox2[d] = apx; a.oy2[d] = apy; a.oz2[d] = apz;
+        a.vox2[d] = apv;
+        if (in_lds) { lx[aL] = avx; ly[aL] = avy; lz[aL] = avz; }
+        else { a.vx2[d] = avx; a.vy2[d] = avy; a.vz2[d] = avz; }
+        if (a.rec_perm) a.rec_perm[d] = (int32_t)key0;
     }
     // more than 64 arrivals: ranks from the keys in LDS, or in the bucket
     for (uint32_t base = 0; !few && base < nb; base += WAVE) {
@@ -1089,6 +1182,116 @@ __device__ __forceinline__ void gc_write_back(const GatherArgs &a, uint32_t c, c
     }
 }
 
+// The write-back fused with the next step's move.  System::step moves right
+// after the previous step's collide (system.cpp:22-47), so a cell's
+// particles, already in their new order with their post-collision velocities
+// in LDS, are moved here exactly as k_move would move them at the next step:
+// the same draws (step + 1, the particle's index in the new order) and the
+// fast path for start and landing voxel empty in the cell's tile.  Every
+// other particle is listed in the cell's range of slow_idx for k_move_slow
+// (the wall path inline would double this kernel's registers), and each
+// leaver in the cell's new index range of the mover lists for the next sort.
+// The moments are taken before the move, as in gc_write_back.
+__device__ __forceinline__ void gc_write_back_move(const GatherArgs &a, uint32_t c, const GcMeta &m,
+                                                   const Member &M, const float *lx, const float *ly,
+                                                   const float *lz, uint32_t *mark, uint32_t K) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t ns = m.ns, n = m.ne - m.ns;
+    const bool in_lds = n <= K;
+    const bool active = m.coeff > 0.0;
+    // the cell's voxel tile in the mark table's LDS (free after the NTC loop)
+    uint8_t *tile = (uint8_t *)mark;
+    int cx, cy, cz;
+    cell_coords(M, c - M.cell0, cx, cy, cz);
+    VoxelView V;
+    load_tile(M, a.types + M.vox0, tile, cx, cy, cz, V);
+    // positions and voxels stored by other lanes in the gather, the tile
+    // stored by this wave: visible after the fence
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    double mm[7] = {0, 0, 0, 0, 0, 0, 0};
+    int32_t per[3] = {0, 0, 0};
+    uint32_t leavers = 0, nslow = 0;
+    // positions and voxels one chunk ahead (lanes past the end re-read the
+    // last particle)
+    uint32_t po = (ns + (lane < n ? lane : n - 1)) << 2;
+    float nox = at_off(a.ox2, po), noy = at_off(a.oy2, po), noz = at_off(a.oz2, po);
+    uint32_t nvp = at_off(a.vox2, po);
+    for (uint32_t base = 0; base < n; base += WAVE) {
+        const uint32_t i = base + lane;
+        const bool valid = i < n;
+        const uint32_t ic = valid ? i : n - 1;
+        const uint32_t p = ns + ic;
+        float ox = nox, oy = noy, oz = noz;
+        const uint32_t vp = nvp;
+        {
+            const uint32_t in = base + WAVE + lane;
+            const uint32_t pn = (ns + (in < n ? in : n - 1)) << 2;
+            nox = at_off(a.ox2, pn); noy = at_off(a.oy2, pn); noz = at_off(a.oz2, pn);
+            nvp = at_off(a.vox2, pn);
+        }
+        po = p << 2;
+        float vx, vy, vz;
+        if (in_lds) {
+            vx = lx[ic]; vy = ly[ic]; vz = lz[ic];
+        } else {
+            vx = GlobalStore::ldf(a.vx2 + p); vy = GlobalStore::ldf(a.vy2 + p); vz = GlobalStore::ldf(a.vz2 + p);
+        }
+        if (valid) {
+            const double dx = vx, dy = vy, dz = vz;
+            mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
+            mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+        }
+        const float cvx = vx, cvy = vy, cvz = vz;
+        int ix, iy, iz;
+        unpack_vox(vp, ix, iy, iz);
+        MoveCounters sc = {0, 0, 0, {0, 0, 0}};
+        const bool fast = move_fast(M, V, ix, iy, iz, ox, oy, oz, vx, vy, vz, sc) & valid;
+        uint32_t newc = c;
+        if (fast) {
+            per[0] += sc.periodic[0];
+            per[1] += sc.periodic[1];
+            per[2] += sc.periodic[2];
+            at_off(a.ox2, po) = ox; at_off(a.oy2, po) = oy; at_off(a.oz2, po) = oz;
+            at_off(a.vx2, po) = vx; at_off(a.vy2, po) = vy; at_off(a.vz2, po) = vz;
+            const uint32_t vpn = pack_vox(ix, iy, iz);
+            if (vpn != vp) at_off(a.vox2, po) = vpn;
+            newc = vpn == vp ? c : cell_of(M, ix, iy, iz);
+        } else if (valid) {
+            // the wall path starts from the post-collision state in memory
+            at_off(a.vx2, po) = cvx; at_off(a.vy2, po) = cvy; at_off(a.vz2, po) = cvz;
+        }
+        const bool slow = valid && !fast;
+        const uint64_t sm = __ballot(slow);
+        if (slow) at_off(a.slow_idx, (ns + nslow + mbcnt(sm)) << 2) = p;
+        nslow += (uint32_t)__popcll(sm);
+        const bool mover = fast && newc != c;
+        const uint64_t mv = __ballot(mover);
+        if (mover) {
+            const uint32_t so = (ns + leavers + mbcnt(mv)) << 2;
+            at_off(a.mv_idx, so) = p;
+            at_off(a.mv_cell, so) = newc;
+            atomicAdd(&a.arrive[newc], 1u);
+        }
+        leavers += (uint32_t)__popcll(mv);
+    }
+    const int32_t p0 = wave_sum(per[0]), p1 = wave_sum(per[1]), p2 = wave_sum(per[2]);
+    if (lane == 0) {
+        a.leave[c] = leavers;
+        a.slow_count[c] = nslow;
+        if (p0) atomicAdd(&a.counters[CNT_PER0], (unsigned long long)(long long)p0);
+        if (p1) atomicAdd(&a.counters[CNT_PER1], (unsigned long long)(long long)p1);
+        if (p2) atomicAdd(&a.counters[CNT_PER2], (unsigned long long)(long long)p2);
+    }
+    if (a.do_sample && active) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) mm[q] = wave_sum(mm[q]);
+        double mine = mm[0];
+#pragma unroll
+        for (int q = 1; q < 7; q++) mine = lane == (uint32_t)q ? mm[q] : mine;
+        if (lane < 7) unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + c, mine);
+    }
+}
+
 // The fused pass as persistent waves on the work queue (as k_collide): the
 // metadata of a chunk of cells arrives one cell per lane, one chunk ahead,
 // so no cell starts with a chain of dependent metadata loads.
@@ -1115,7 +1318,8 @@ __device__ __forceinline__ GcBatch load_gc_batch(const GatherArgs &a, uint32_t c
     return b;
 }
 
-template <int GC_BATCH>
+// MOVE: with the next step's move (gc_write_back_move).
+template <int GC_BATCH, bool MOVE>
 __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
     const uint32_t K = a.lds_cap;
     float *lx = dyn_lds, *ly = dyn_lds + K, *lz = dyn_lds + 2 * K;
@@ -1155,6 +1359,12 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
         }
         const bool write_back = gc_gather_collide<GC_BATCH>(a, c, m, M, P, lx, ly, lz, mark, K);
         const GcMeta mc = m;
+        if (MOVE) {
+            // the move's registers are not shared with the next cell's batch:
+            // that one is loaded afterwards
+            if (write_back) gc_write_back_move(a, c, mc, M, lx, ly, lz, mark, K);
+            else if (threadIdx.x == 0) { a.leave[c] = 0; a.slow_count[c] = 0; }
+        }
         // the next cell: its first batch and keys go out before this cell's
         // write-back stores, so waiting for them does not wait for those
         if (++j == a.chunk) {
@@ -1169,7 +1379,7 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
             DSMC_GC_META(j);
             gc_load_ahead<GC_BATCH>(a, m, P);
         }
-        if (write_back) gc_write_back(a, c, mc, lx, ly, lz, K);
+        if (!MOVE && write_back) gc_write_back(a, c, mc, lx, ly, lz, K);
     }
 #undef DSMC_GC_META
 }
@@ -1303,6 +1513,7 @@ __global__ void k_begin_step(uint32_t *step, uint32_t *mv_count) {
         mv_count[0] = 0u;
         mv_count[1] = 0u;
         mv_count[2] = 0u;
+        mv_count[3] = 0u;
     }
 }
 
@@ -1386,7 +1597,8 @@ hipError_t launch_gather(const GatherArgs &a, uint32_t ncells, hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t chunk_opt, hipStream_t s) {
+hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t chunk_opt, bool move_ahead,
+                                 hipStream_t s) {
     if (ncells == 0) return hipSuccess;
     const size_t lds = (size_t)a.lds_cap * 12 + MARKS * 4;
     int dev = 0, cus = 0, per_cu = 0;
@@ -1397,8 +1609,9 @@ hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t 
     // LDS for at most three waves per SIMD (12 per CU): the large batch
     // costs no occupancy
     const bool large = lds * 16 > (size_t)160 * 1024;
-    e = large ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_collide<GC_BATCH_LARGE>, 64, lds)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_collide<GC_BATCH_SMALL>, 64, lds);
+    auto kern = large ? (move_ahead ? k_gather_collide<GC_BATCH_LARGE_MOVE, true> : k_gather_collide<GC_BATCH_LARGE, false>)
+                      : (move_ahead ? k_gather_collide<GC_BATCH_SMALL, true> : k_gather_collide<GC_BATCH_SMALL, false>);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds);
     if (e != hipSuccess) return e;
     uint32_t grid = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
     GatherArgs b = a;
@@ -1407,16 +1620,20 @@ hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t 
     if (!chunk_opt && b.chunk > 8u) b.chunk = 8u;
     const uint32_t nchunks = (ncells + b.chunk - 1) / b.chunk;
     if (grid > nchunks) grid = nchunks;
-    if (large)
-        hipLaunchKernelGGL(k_gather_collide<GC_BATCH_LARGE>, dim3(grid), dim3(64), lds, s, b);
-    else
-        hipLaunchKernelGGL(k_gather_collide<GC_BATCH_SMALL>, dim3(grid), dim3(64), lds, s, b);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, s, b);
     LAUNCH_CHECK();
     if (a.do_collide) {
         hipLaunchKernelGGL(k_cell_totals, dim3(256), dim3(256), 0, s, a.stat_pairs, a.stat_acc, a.start_new,
                            a.coeff, a.ncells, a.lds_cap, a.counters);
         LAUNCH_CHECK();
     }
+    return hipSuccess;
+}
+
+hipError_t launch_move_slow(const MoveArgs &a, uint32_t ncells, hipStream_t s) {
+    if (ncells == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_move_slow, dim3((ncells + WAVE - 1) / WAVE), dim3(64), 0, s, a);
+    LAUNCH_CHECK();
     return hipSuccess;
 }
 

@@ -24,7 +24,7 @@ I64 = ctypes.c_int64
 
 WALL_THERMAL, WALL_SPECULAR, WALL_CERCIGNANI_LAMPIS = 0, 1, 2
 OPT_PHASES, OPT_RECORD, OPT_TOTALS, OPT_LDS_CAPACITY, OPT_USE_GRAPH, OPT_TIMING = 1, 2, 3, 4, 5, 6
-OPT_COLLIDE_CHUNK, OPT_GRAPH_STEPS, OPT_FUSED, OPT_MOVE_QUEUE = 7, 8, 9, 10
+OPT_COLLIDE_CHUNK, OPT_GRAPH_STEPS, OPT_FUSED, OPT_MOVE_QUEUE, OPT_MOVE_AHEAD = 7, 8, 9, 10, 11
 PHASE_MOVE, PHASE_SORT, PHASE_COLLIDE = 1, 2, 4
 
 _STATUS = {0: "OK", 1: "invalid argument", 2: "configuration", 3: "capacity", 4: "HIP runtime",
