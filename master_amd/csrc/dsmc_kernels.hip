@@ -49,17 +49,28 @@ __device__ __forceinline__ void load_tile(const Member &M, const uint8_t *types,
     uint32_t a = i / TYZ, b = (i - a * TYZ) / TZ, c = i - a * TYZ - b * TZ;
     const uint32_t da = WAVE / TYZ, db = (WAVE - da * TYZ) / TZ, dc = WAVE - da * TYZ - db * TZ;
     const uint32_t n1 = (uint32_t)M.n[1], n2 = (uint32_t)M.n[2];
-    for (; i < tot; i += WAVE) {
-        const uint32_t x = (uint32_t)wrapi(V.tx0 + (int)a, M.n[0]);
-        const uint32_t y = (uint32_t)wrapi(V.ty0 + (int)b, M.n[1]);
-        const uint32_t z = (uint32_t)wrapi(V.tz0 + (int)c, M.n[2]);
-        // axes of at most 1024 voxels: 24-bit products, index below 2^30
-        tile[i] = types[__umul24(__umul24(x, n1) + y, n2) + z];
-        c += dc;
-        b += db;
-        a += da;
-        if (c >= TZ) { c -= TZ; b++; }
-        if (b >= TY) { b -= TY; a++; }
+    // four voxels per lane in flight per round trip (a 6^3 tile: one);
+    // lanes past the tile read voxel 0 and store nothing
+    for (uint32_t i0 = 0; i0 < tot; i0 += 4 * WAVE) {
+        uint32_t t[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t x = (uint32_t)wrapi(V.tx0 + (int)a, M.n[0]);
+            const uint32_t y = (uint32_t)wrapi(V.ty0 + (int)b, M.n[1]);
+            const uint32_t z = (uint32_t)wrapi(V.tz0 + (int)c, M.n[2]);
+            // axes of at most 1024 voxels: 24-bit products, index below 2^30
+            const bool in = i + k * WAVE < tot;
+            t[k] = types[in ? __umul24(__umul24(x, n1) + y, n2) + z : 0u];
+            c += dc;
+            b += db;
+            a += da;
+            if (c >= TZ) { c -= TZ; b++; }
+            if (b >= TY) { b -= TY; a++; }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (i + k * WAVE < tot) tile[i + k * WAVE] = (uint8_t)t[k];
+        i += 4 * WAVE;
     }
 }
 
