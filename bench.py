@@ -17,9 +17,13 @@ Prints one JSON line (the driver contract) with
   HBM peak.  Algorithmic bytes per particle: k_move 56 (28-byte state read
   and written), k_gather 56 (state read from the old order, written in cell
   order), k_collide 24 (velocities read and written), k_gather_collide 56
-  (the fused gather + collide pass the library runs by default: the collide
-  adds no traffic); the step's reference count is 140 (sort row 60 = gather
-  56 + the 4-byte cell key, read inside k_move; collide 24).
+  (the fused gather + collide pass: the collide adds no traffic) and
+  k_gather_collide_move 56 (the same pass also moving its cells for the next
+  step, which the library runs for all but the last step of a dsmc_step
+  call: the move works on the data the pass has just written); the step's
+  reference count is 140 (sort row 60 = gather 56 + the 4-byte cell key,
+  read inside k_move; collide 24).  The eager per-kernel profile is one
+  dsmc_step call, so k_move runs on its first step only.
   `traffic` is the HBM bytes one launch of that kernel really moves, from two
   rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over the same workload,
   each scaled by what the counter reads for a plain 1 GiB copy in the same
@@ -45,7 +49,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per particle and launch
-KERNEL_BYTES = {"k_move": 56, "k_gather": 56, "k_collide": 24, "k_gather_collide": 56}
+KERNEL_BYTES = {"k_move": 56, "k_gather": 56, "k_collide": 24, "k_gather_collide": 56, "k_gather_collide_move": 56}
 STEP_BYTES = 140
 COPY_BYTES = 1 << 30
 
@@ -188,8 +192,13 @@ def pmc_pass(args):
 
 
 def _short(name):
-    m = re.search(r"\b(k_\w+)(?:<[^>]*>)?\(", name)
-    return m.group(1) if m else name
+    m = re.search(r"\b(k_\w+)(<[^>]*>)?\(", name)
+    if not m:
+        return name
+    # the fused pass's instantiation with the next step's move
+    if m.group(1) == "k_gather_collide" and m.group(2) and "true" in m.group(2):
+        return "k_gather_collide_move"
+    return m.group(1)
 
 
 def _read_counter(db, counter, steps):
@@ -280,7 +289,7 @@ def main():
     ap.add_argument("--statistics", type=int, default=0,
                     help="also run the statistics sampler every N steps (100 y bins); off by default")
     ap.add_argument("--pmc-pass", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--pmc-steps", type=int, default=2, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-steps", type=int, default=3, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-timeout", type=int, default=150, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -676,7 +676,7 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample, bool move_prev = false, bo
         // were read by the scan)
         if (move_next) HIPCHK(h, launch_fill_u32(h->d_arrive, 0u, h->ncells, s));
         {
-            Scope sc(h, "k_gather_collide");
+            Scope sc(h, move_next ? "k_gather_collide_move" : "k_gather_collide");
             HIPCHK(h, launch_gather_collide(gather_args(h, collide, sample), h->ncells, h->collide_chunk, move_next, s));
         }
         h->cur = 1 - h->cur;
