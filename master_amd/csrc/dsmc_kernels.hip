@@ -526,8 +526,9 @@ static constexpr int GATHER_BATCH = 8;      // chunks of 64 particles per round 
 // three waves per SIMD anyway (see launch_gather_collide).  The next cell's
 // batch is in registers during a cell's write-back.
 static constexpr int GC_BATCH_SMALL = 5, GC_BATCH_LARGE = 11;
-// with the next step's move in the pass (gc_write_back_move): 112 and 165 VGPRs
-static constexpr int GC_BATCH_LARGE_MOVE = 8;
+// With the next step's move in the pass (gc_write_back_move) the batch holds
+// no positions: 5 (96 VGPRs) and 14 (153 VGPRs).
+static constexpr int GC_BATCH_LARGE_MOVE = 14;
 
 __global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
     const uint32_t c = blockIdx.x + a.cell_begin;
@@ -971,44 +972,51 @@ struct GcMeta {
 
 // A cell's first batch of old particles and its arrival keys, loaded one
 // cell ahead (k_gather_collide issues them before the previous cell's
-// write-back stores).
-template <int GC_BATCH>
+// write-back stores).  POS: with the stayers' positions, which the gather
+// copies to the new buffer; with the move fused in (gc_write_back_move) it
+// reads them from the old buffer itself and the gather needs none.
+template <int GC_BATCH, bool POS>
 struct GcAhead {
-    float ox[GC_BATCH], oy[GC_BATCH], oz[GC_BATCH], vx[GC_BATCH], vy[GC_BATCH], vz[GC_BATCH];
+    static constexpr int NP = POS ? GC_BATCH : 1;
+    float ox[NP], oy[NP], oz[NP];
+    float vx[GC_BATCH], vy[GC_BATCH], vz[GC_BATCH];
     uint32_t vp[GC_BATCH];
     uint32_t key0;
 };
 
-template <int GC_BATCH>
-__device__ __forceinline__ void gc_load_batch(const GatherArgs &a, uint32_t base, uint32_t oe, GcAhead<GC_BATCH> &P) {
+template <int GC_BATCH, bool POS>
+__device__ __forceinline__ void gc_load_batch(const GatherArgs &a, uint32_t base, uint32_t oe,
+                                              GcAhead<GC_BATCH, POS> &P) {
     const uint32_t lane = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < GC_BATCH; q++) {
         const uint32_t p = base + q * WAVE + lane;
         // every field written on every lane: no earlier value stays live
         P.vp[q] = 0xffffffffu;
-        P.ox[q] = P.oy[q] = P.oz[q] = P.vx[q] = P.vy[q] = P.vz[q] = 0.0f;
+        P.vx[q] = P.vy[q] = P.vz[q] = 0.0f;
+        if constexpr (POS) P.ox[q] = P.oy[q] = P.oz[q] = 0.0f;
         if (p < oe) {
             P.vp[q] = a.vox[p];
-            P.ox[q] = a.ox[p]; P.oy[q] = a.oy[p]; P.oz[q] = a.oz[p];
+            if constexpr (POS) { P.ox[q] = a.ox[p]; P.oy[q] = a.oy[p]; P.oz[q] = a.oz[p]; }
             P.vx[q] = a.vx[p]; P.vy[q] = a.vy[p]; P.vz[q] = a.vz[p];
         }
     }
 }
 
-template <int GC_BATCH>
-__device__ __forceinline__ void gc_load_ahead(const GatherArgs &a, const GcMeta &m, GcAhead<GC_BATCH> &P) {
-    gc_load_batch<GC_BATCH>(a, m.os, m.oe, P);
+template <int GC_BATCH, bool POS>
+__device__ __forceinline__ void gc_load_ahead(const GatherArgs &a, const GcMeta &m, GcAhead<GC_BATCH, POS> &P) {
+    gc_load_batch<GC_BATCH, POS>(a, m.os, m.oe, P);
     const uint32_t nb = m.be - m.bs;
     P.key0 = threadIdx.x < nb ? a.bucket[m.bs + threadIdx.x] : 0xffffffffu;
 }
 
 // Gather (the first batch and the keys already loaded in P) and collide one
-// cell; returns whether its write-back (gc_write_back) is due.
-template <int GC_BATCH>
+// cell; returns whether its write-back (gc_write_back) is due and sets
+// n_low, the number of arrivals placed before the stayers.
+template <int GC_BATCH, bool MOVE>
 __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t c, const GcMeta &m,
-                                                  const Member &M, GcAhead<GC_BATCH> &P, float *lx, float *ly,
-                                                  float *lz, uint32_t *mark, uint32_t K) {
+                                                  const Member &M, GcAhead<GC_BATCH, !MOVE> &P, float *lx, float *ly,
+                                                  float *lz, uint32_t *mark, uint32_t K, uint32_t &n_low_out) {
     const uint32_t lane = threadIdx.x;
     // the mark table is free until the NTC loop: it holds the arrival keys
     uint32_t *keys = mark;
@@ -1016,6 +1024,7 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
     const uint32_t n = ne - ns;
     const double coeff = m.coeff;
     const bool active = coeff > 0.0;
+    n_low_out = 0;
     if (n == 0 && !(active && a.do_collide)) return false;
     const uint32_t os = m.os, oe = m.oe;
     const uint32_t bs = m.bs, be = m.be;
@@ -1065,7 +1074,7 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
     }
     uint32_t run = 0;
     for (uint32_t base = os; base < oe; base += GC_BATCH * WAVE) {
-        if (base != os) gc_load_batch<GC_BATCH>(a, base, oe, P);
+        if (base != os) gc_load_batch<GC_BATCH, !MOVE>(a, base, oe, P);
 #pragma unroll
         for (int q = 0; q < GC_BATCH; q++) {
             const uint32_t p = base + q * WAVE + lane;
@@ -1079,8 +1088,10 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
             if (stay) {
                 const uint32_t L = n_low + run + mbcnt(sm);
                 const uint32_t d = ns + L;
-                a.ox2[d] = P.ox[q]; a.oy2[d] = P.oy[q]; a.oz2[d] = P.oz[q];
-                a.vox2[d] = P.vp[q];
+                if constexpr (!MOVE) {
+                    a.ox2[d] = P.ox[q]; a.oy2[d] = P.oy[q]; a.oz2[d] = P.oz[q];
+                    a.vox2[d] = P.vp[q];
+                }
                 if (in_lds) { lx[L] = P.vx[q]; ly[L] = P.vy[q]; lz[L] = P.vz[q]; }
                 else { a.vx2[d] = P.vx[q]; a.vy2[d] = P.vy[q]; a.vz2[d] = P.vz[q]; }
                 if (a.rec_perm) a.rec_perm[d] = (int32_t)p;
@@ -1130,6 +1141,7 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
     __syncthreads();
 
     // ---- collide (k_collide)
+    n_low_out = n_low;
     if (active && a.do_collide) {
         const uint32_t step = *a.step;
         const double vrm = m.vrm;
@@ -1193,98 +1205,172 @@ __device__ __forceinline__ void gc_write_back(const GatherArgs &a, uint32_t c, c
     }
 }
 
+// A particle's position and voxel, loaded ahead of its move.
+struct GcPos {
+    float x, y, z;
+    uint32_t v;
+};
+
+__device__ __forceinline__ GcPos gc_load_pos(const float *ox, const float *oy, const float *oz, const uint32_t *vox,
+                                             uint32_t q) {
+    const uint32_t o = q << 2;
+    return GcPos{at_off(ox, o), at_off(oy, o), at_off(oz, o), at_off(vox, o)};
+}
+
+// Moves one particle per valid lane for the next step, as k_move would: the
+// particle goes to index p = ns + L of the new buffer, its post-collision
+// velocity is at slot L of the cell (LDS, or the new buffer).  Every field
+// is stored: the gather leaves the positions of a cell held in LDS
+// unwritten.  Particles off the fast path keep their position and
+// post-collision velocity and are listed for k_move_slow; fast movers are
+// listed for the next sort.
+template <bool IN_LDS>
+__device__ __forceinline__ void gc_move_one(const GatherArgs &a, uint32_t c, uint32_t ns, uint32_t L, bool valid,
+                                            const Member &M, const VoxelView &V, const float *lx, const float *ly,
+                                            const float *lz, const GcPos &P, int32_t *per, uint32_t &leavers,
+                                            uint32_t &nslow) {
+    const uint32_t p = ns + L;
+    const uint32_t po = p << 2;
+    float ox = P.x, oy = P.y, oz = P.z;
+    float vx, vy, vz;
+    if (IN_LDS) {
+        vx = lx[L]; vy = ly[L]; vz = lz[L];
+    } else {
+        vx = GlobalStore::ldf(a.vx2 + p); vy = GlobalStore::ldf(a.vy2 + p); vz = GlobalStore::ldf(a.vz2 + p);
+    }
+    const float cvx = vx, cvy = vy, cvz = vz;
+    int ix, iy, iz;
+    unpack_vox(P.v, ix, iy, iz);
+    MoveCounters sc = {0, 0, 0, {0, 0, 0}};
+    const bool fast = move_fast(M, V, ix, iy, iz, ox, oy, oz, vx, vy, vz, sc) & valid;
+    uint32_t newc = c;
+    if (fast) {
+        per[0] += sc.periodic[0];
+        per[1] += sc.periodic[1];
+        per[2] += sc.periodic[2];
+        const uint32_t vpn = pack_vox(ix, iy, iz);
+        at_off(a.ox2, po) = ox; at_off(a.oy2, po) = oy; at_off(a.oz2, po) = oz;
+        at_off(a.vx2, po) = vx; at_off(a.vy2, po) = vy; at_off(a.vz2, po) = vz;
+        at_off(a.vox2, po) = vpn;
+        newc = vpn == P.v ? c : cell_of(M, ix, iy, iz);
+    } else if (valid) {
+        at_off(a.ox2, po) = P.x; at_off(a.oy2, po) = P.y; at_off(a.oz2, po) = P.z;
+        at_off(a.vx2, po) = cvx; at_off(a.vy2, po) = cvy; at_off(a.vz2, po) = cvz;
+        at_off(a.vox2, po) = P.v;
+    }
+    const bool slow = valid && !fast;
+    const uint64_t sm = __ballot(slow);
+    if (slow) at_off(a.slow_idx, (ns + nslow + mbcnt(sm)) << 2) = p;
+    nslow += (uint32_t)__popcll(sm);
+    const bool mover = fast && newc != c;
+    const uint64_t mv = __ballot(mover);
+    if (mover) {
+        const uint32_t so = (ns + leavers + mbcnt(mv)) << 2;
+        at_off(a.mv_idx, so) = p;
+        at_off(a.mv_cell, so) = newc;
+        atomicAdd(&a.arrive[newc], 1u);
+    }
+    leavers += (uint32_t)__popcll(mv);
+}
+
+// The move of gc_write_back_move.  Stayers come from the old buffer (which
+// this pass does not write), one chunk ahead, the gather's classification
+// repeated to place them; arrivals, slots [0, n_low) and
+// [n_low + n_stay, n), from the new buffer, where the gather stored them.
+// IN_LDS: the velocities' source (a merged path would make the compiler wait
+// for every load in flight, the prefetch included, before the first use).
+template <bool IN_LDS>
+__device__ __forceinline__ void gc_move_cell(const GatherArgs &a, uint32_t c, uint32_t ns, uint32_t os,
+                                             uint32_t no, uint32_t nb, uint32_t n_low, uint32_t n_stay,
+                                             const Member &M, const VoxelView &V, const float *lx,
+                                             const float *ly, const float *lz, GcPos A, int32_t *per,
+                                             uint32_t &leavers, uint32_t &nslow) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t run = 0;
+    for (uint32_t base = 0; base < no; base += WAVE) {
+        const uint32_t i = base + lane;
+        const GcPos cur = A;
+        A = gc_load_pos(a.ox, a.oy, a.oz, a.vox, os + (i + WAVE < no ? i + WAVE : no - 1));
+        bool stay = false;
+        if (i < no) {
+            int ix, iy, iz;
+            unpack_vox(cur.v, ix, iy, iz);
+            stay = cell_of(M, ix, iy, iz) == c;
+        }
+        const uint64_t sm = __ballot(stay);
+        const uint32_t L = stay ? n_low + run + mbcnt(sm) : 0u;
+        gc_move_one<IN_LDS>(a, c, ns, L, stay, M, V, lx, ly, lz, cur, per, leavers, nslow);
+        run += (uint32_t)__popcll(sm);
+    }
+    for (uint32_t base = 0; base < nb; base += WAVE) {
+        const uint32_t r = base + lane;
+        const bool valid = r < nb;
+        const uint32_t L = valid ? (r < n_low ? r : n_stay + r) : 0u;
+        const GcPos cur = gc_load_pos(a.ox2, a.oy2, a.oz2, a.vox2, ns + L);
+        gc_move_one<IN_LDS>(a, c, ns, L, valid, M, V, lx, ly, lz, cur, per, leavers, nslow);
+    }
+}
+
 // The write-back fused with the next step's move.  System::step moves right
 // after the previous step's collide (system.cpp:22-47), so a cell's
-// particles, already in their new order with their post-collision velocities
-// in LDS, are moved here exactly as k_move would move them at the next step:
-// the same draws (step + 1, the particle's index in the new order) and the
-// fast path for start and landing voxel empty in the cell's tile.  Every
+// particles, already in their new order with their post-collision
+// velocities, are moved here exactly as k_move would move them at the next
+// step: the same draws (step + 1, the particle's index in the new order) and
+// the fast path for start and landing voxel empty in the cell's tile.  Every
 // other particle is listed in the cell's range of slow_idx for k_move_slow
 // (the wall path inline would double this kernel's registers), and each
 // leaver in the cell's new index range of the mover lists for the next sort.
-// The moments are taken before the move, as in gc_write_back.
+// The gather stored only the arrivals' positions: each particle's state is
+// written to the new buffer once, here.  The moments are gc_write_back's,
+// summed in the same order.
 __device__ __forceinline__ void gc_write_back_move(const GatherArgs &a, uint32_t c, const GcMeta &m,
-                                                   const Member &M, const float *lx, const float *ly,
-                                                   const float *lz, uint32_t *mark, uint32_t K) {
+                                                   uint32_t n_low, const Member &M, const float *lx,
+                                                   const float *ly, const float *lz, uint32_t *mark,
+                                                   uint32_t K) {
     const uint32_t lane = threadIdx.x;
     const uint32_t ns = m.ns, n = m.ne - m.ns;
+    const uint32_t os = m.os, no = m.oe - m.os;
+    const uint32_t nb = m.be - m.bs, n_stay = n - nb;
     const bool in_lds = n <= K;
     const bool active = m.coeff > 0.0;
+    // the first stayer chunk goes out before the tile load
+    GcPos A{0.0f, 0.0f, 0.0f, 0u};
+    if (no) A = gc_load_pos(a.ox, a.oy, a.oz, a.vox, os + (lane < no ? lane : no - 1));
     // the cell's voxel tile in the mark table's LDS (free after the NTC loop)
     uint8_t *tile = (uint8_t *)mark;
     int cx, cy, cz;
     cell_coords(M, c - M.cell0, cx, cy, cz);
     VoxelView V;
     load_tile(M, a.types + M.vox0, tile, cx, cy, cz, V);
-    // positions and voxels stored by other lanes in the gather, the tile
-    // stored by this wave: visible after the fence
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-    double mm[7] = {0, 0, 0, 0, 0, 0, 0};
-    int32_t per[3] = {0, 0, 0};
-    uint32_t leavers = 0, nslow = 0;
-    // positions and voxels one chunk ahead (lanes past the end re-read the
-    // last particle)
-    uint32_t po = (ns + (lane < n ? lane : n - 1)) << 2;
-    float nox = at_off(a.ox2, po), noy = at_off(a.oy2, po), noz = at_off(a.oz2, po);
-    uint32_t nvp = at_off(a.vox2, po);
-    for (uint32_t base = 0; base < n; base += WAVE) {
-        const uint32_t i = base + lane;
-        const bool valid = i < n;
-        const uint32_t ic = valid ? i : n - 1;
-        const uint32_t p = ns + ic;
-        float ox = nox, oy = noy, oz = noz;
-        const uint32_t vp = nvp;
-        {
-            const uint32_t in = base + WAVE + lane;
-            const uint32_t pn = (ns + (in < n ? in : n - 1)) << 2;
-            nox = at_off(a.ox2, pn); noy = at_off(a.oy2, pn); noz = at_off(a.oz2, pn);
-            nvp = at_off(a.vox2, pn);
-        }
-        po = p << 2;
-        float vx, vy, vz;
-        if (in_lds) {
-            vx = lx[ic]; vy = ly[ic]; vz = lz[ic];
-        } else {
-            vx = GlobalStore::ldf(a.vx2 + p); vy = GlobalStore::ldf(a.vy2 + p); vz = GlobalStore::ldf(a.vz2 + p);
-        }
-        if (valid) {
-            const double dx = vx, dy = vy, dz = vz;
+    // one wave per block, and a wave's LDS accesses complete in order: the
+    // tile is visible to every lane without a barrier
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (a.do_sample && active) {
+        // before the move, which changes the velocities it stores; per lane
+        // in slot order, as gc_write_back
+        double mm[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t i = lane; i < n; i += WAVE) {
+            double dx, dy, dz;
+            if (in_lds) {
+                dx = lx[i]; dy = ly[i]; dz = lz[i];
+            } else {
+                dx = GlobalStore::ldf(a.vx2 + ns + i); dy = GlobalStore::ldf(a.vy2 + ns + i);
+                dz = GlobalStore::ldf(a.vz2 + ns + i);
+            }
             mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
             mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
         }
-        const float cvx = vx, cvy = vy, cvz = vz;
-        int ix, iy, iz;
-        unpack_vox(vp, ix, iy, iz);
-        MoveCounters sc = {0, 0, 0, {0, 0, 0}};
-        const bool fast = move_fast(M, V, ix, iy, iz, ox, oy, oz, vx, vy, vz, sc) & valid;
-        uint32_t newc = c;
-        if (fast) {
-            per[0] += sc.periodic[0];
-            per[1] += sc.periodic[1];
-            per[2] += sc.periodic[2];
-            at_off(a.ox2, po) = ox; at_off(a.oy2, po) = oy; at_off(a.oz2, po) = oz;
-            at_off(a.vx2, po) = vx; at_off(a.vy2, po) = vy; at_off(a.vz2, po) = vz;
-            const uint32_t vpn = pack_vox(ix, iy, iz);
-            if (vpn != vp) at_off(a.vox2, po) = vpn;
-            newc = vpn == vp ? c : cell_of(M, ix, iy, iz);
-        } else if (valid) {
-            // the wall path starts from the post-collision state in memory
-            at_off(a.vx2, po) = cvx; at_off(a.vy2, po) = cvy; at_off(a.vz2, po) = cvz;
-        }
-        const bool slow = valid && !fast;
-        const uint64_t sm = __ballot(slow);
-        if (slow) at_off(a.slow_idx, (ns + nslow + mbcnt(sm)) << 2) = p;
-        nslow += (uint32_t)__popcll(sm);
-        const bool mover = fast && newc != c;
-        const uint64_t mv = __ballot(mover);
-        if (mover) {
-            const uint32_t so = (ns + leavers + mbcnt(mv)) << 2;
-            at_off(a.mv_idx, so) = p;
-            at_off(a.mv_cell, so) = newc;
-            atomicAdd(&a.arrive[newc], 1u);
-        }
-        leavers += (uint32_t)__popcll(mv);
+#pragma unroll
+        for (int q = 0; q < 7; q++) mm[q] = wave_sum(mm[q]);
+        double mine = mm[0];
+#pragma unroll
+        for (int q = 1; q < 7; q++) mine = lane == (uint32_t)q ? mm[q] : mine;
+        if (lane < 7) unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + c, mine);
     }
+    int32_t per[3] = {0, 0, 0};
+    uint32_t leavers = 0, nslow = 0;
+    if (in_lds) gc_move_cell<true>(a, c, ns, os, no, nb, n_low, n_stay, M, V, lx, ly, lz, A, per, leavers, nslow);
+    else gc_move_cell<false>(a, c, ns, os, no, nb, n_low, n_stay, M, V, lx, ly, lz, A, per, leavers, nslow);
     const int32_t p0 = wave_sum(per[0]), p1 = wave_sum(per[1]), p2 = wave_sum(per[2]);
     if (lane == 0) {
         a.leave[c] = leavers;
@@ -1292,14 +1378,6 @@ __device__ __forceinline__ void gc_write_back_move(const GatherArgs &a, uint32_t
         if (p0) atomicAdd(&a.counters[CNT_PER0], (unsigned long long)(long long)p0);
         if (p1) atomicAdd(&a.counters[CNT_PER1], (unsigned long long)(long long)p1);
         if (p2) atomicAdd(&a.counters[CNT_PER2], (unsigned long long)(long long)p2);
-    }
-    if (a.do_sample && active) {
-#pragma unroll
-        for (int q = 0; q < 7; q++) mm[q] = wave_sum(mm[q]);
-        double mine = mm[0];
-#pragma unroll
-        for (int q = 1; q < 7; q++) mine = lane == (uint32_t)q ? mm[q] : mine;
-        if (lane < 7) unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + c, mine);
     }
 }
 
@@ -1357,10 +1435,10 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
         mi = __builtin_amdgcn_readlane(cur.member, (J));                 \
     } while (0)
     uint64_t c64 = (uint64_t)q0 * a.chunk;
-    GcAhead<GC_BATCH> P;
+    GcAhead<GC_BATCH, !MOVE> P;
     if (c64 < a.ncells) {
         DSMC_GC_META(0);
-        gc_load_ahead<GC_BATCH>(a, m, P);
+        gc_load_ahead<GC_BATCH, !MOVE>(a, m, P);
     }
     while (c64 < a.ncells) {
         const uint32_t c = (uint32_t)c64;
@@ -1368,12 +1446,13 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
             M = a.members[mi];
             member = mi;
         }
-        const bool write_back = gc_gather_collide<GC_BATCH>(a, c, m, M, P, lx, ly, lz, mark, K);
+        uint32_t n_low;
+        const bool write_back = gc_gather_collide<GC_BATCH, MOVE>(a, c, m, M, P, lx, ly, lz, mark, K, n_low);
         const GcMeta mc = m;
         if (MOVE) {
             // the move's registers are not shared with the next cell's batch:
             // that one is loaded afterwards
-            if (write_back) gc_write_back_move(a, c, mc, M, lx, ly, lz, mark, K);
+            if (write_back) gc_write_back_move(a, c, mc, n_low, M, lx, ly, lz, mark, K);
             else if (threadIdx.x == 0) { a.leave[c] = 0; a.slow_count[c] = 0; }
         }
         // the next cell: its first batch and keys go out before this cell's
@@ -1388,7 +1467,7 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
         c64 = (uint64_t)q0 * a.chunk + j;
         if (c64 < a.ncells) {
             DSMC_GC_META(j);
-            gc_load_ahead<GC_BATCH>(a, m, P);
+            gc_load_ahead<GC_BATCH, !MOVE>(a, m, P);
         }
         if (!MOVE && write_back) gc_write_back(a, c, mc, lx, ly, lz, K);
     }

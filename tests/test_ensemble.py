@@ -67,7 +67,8 @@ def test_tuning_knobs_leave_results_unchanged(gpu):
     """Collide chunking, graph length, the fused gather + collide pass, the
     mover's wall-path queue (0: every wall-path particle goes through the
     overflow pass), the LDS capacity (1024: the fused pass's large-batch
-    variant), moving ahead (the fused pass moving for the next step, on by
+    variant; 96: most cells on the fused pass's global-memory path, with the
+    move fused in or not), moving ahead (the fused pass moving for the next step, on by
     default; eager chains and graphs) and the autotuner's choice change
     launch shapes only: trajectories stay bit-identical."""
     dsmc = gpu
@@ -92,7 +93,8 @@ def test_tuning_knobs_leave_results_unchanged(gpu):
                   {dsmc.OPT_COLLIDE_CHUNK: 16, dsmc.OPT_GRAPH_STEPS: 8},
                   {dsmc.OPT_FUSED: 0}, {dsmc.OPT_FUSED: 0, dsmc.OPT_USE_GRAPH: 0},
                   {dsmc.OPT_MOVE_QUEUE: 0}, {dsmc.OPT_MOVE_QUEUE: 3, dsmc.OPT_USE_GRAPH: 0},
-                  {dsmc.OPT_LDS_CAPACITY: 1024}, {dsmc.OPT_MOVE_AHEAD: 0},
+                  {dsmc.OPT_LDS_CAPACITY: 1024}, {dsmc.OPT_LDS_CAPACITY: 96},
+                  {dsmc.OPT_LDS_CAPACITY: 96, dsmc.OPT_USE_GRAPH: 0}, {dsmc.OPT_MOVE_AHEAD: 0},
                   {dsmc.OPT_MOVE_AHEAD: 0, dsmc.OPT_USE_GRAPH: 0}, {dsmc.OPT_USE_GRAPH: 0},
                   {dsmc.OPT_GRAPH_STEPS: 16}):
         g = make()
