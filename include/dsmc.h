@@ -95,11 +95,26 @@ typedef struct {
 
 typedef struct dsmc_handle dsmc_handle;
 
+/* The run and output keys of dsmc.ini that main.cpp and DSMC_IO read
+ * (settings.cpp:6-14); booleans are true only for the text "true", as
+ * CIniFile::getbool (cinifile.h:73-82) reads them. */
+typedef struct {
+    int timesteps;                  /* settings timesteps (main.cpp:57)      */
+    int load_previous_state;        /* settings load_previous_state          */
+    int create_movie;               /* settings create_movie                 */
+    int movie_every_n_frame;        /* settings movie_every_n_frame          */
+    int movie_molecules;            /* settings movie_molecules              */
+    int statistics_interval;        /* settings statistics_interval          */
+    int sampling_bins;              /* settings sampling_bins                */
+} dsmc_run_settings;
+
 /* --- construction: System::initialize (system.cpp:241-382) ------------- */
 /* Parse a reference dsmc.ini (settings.cpp:3-44, cinifile.h:28-98) and apply
  * the unit conversion of System::initialize.  world_dir (optional) receives
  * the `world` key. */
 dsmc_status dsmc_settings_from_ini(const char *path, dsmc_settings *out, char *world_dir, int world_dir_len);
+/* The run and output keys of the same file (dsmc_run_settings). */
+dsmc_status dsmc_run_settings_from_ini(const char *path, dsmc_run_settings *out);
 /* Read a one-rank world file "<dir>/0000.bin" (dsmc_io.cpp:183-220) into
  * library-owned arrays; release with dsmc_world_free. */
 dsmc_status dsmc_world_read(const char *path, dsmc_world *out);
@@ -134,6 +149,23 @@ dsmc_status dsmc_get_state(dsmc_handle *h, double *rv6);
 dsmc_status dsmc_num_particles(dsmc_handle *h, int64_t *n);
 dsmc_status dsmc_save_state_file(dsmc_handle *h, const char *path);
 dsmc_status dsmc_load_state_file(dsmc_handle *h, const char *path);
+
+/* --- movie: DSMC_IO::save_state_to_movie_file (dsmc_io.cpp:50-80), which
+ * main.cpp:57-59 calls before every step.  With a movie set, dsmc_step writes
+ * a frame to `path` before each step whose step count is a multiple of
+ * every_n_steps: an int32 count (= molecules, the reference's
+ * movie_molecules on one rank) and count rows of r0 r1 r2 v0 v1 v2 as
+ * float32 (r absolute, reduced units, as dsmc_get_state).  The rows follow
+ * the same particles from frame to frame, as the reference's molecule
+ * storage order does: the first `molecules` particles of the seeded or
+ * loaded order when the movie is set before dsmc_initialize* or
+ * dsmc_load_state*, else of the order at the call.  Rows beyond the particle
+ * count stay zero.  path NULL or every_n_steps <= 0 closes the movie.  While
+ * a movie is set the sort carries particle ids and the fused pass does not
+ * move ahead (trajectories are unchanged). */
+dsmc_status dsmc_set_movie(dsmc_handle *h, const char *path, int every_n_steps, int molecules);
+/* The frame of the current state (molecules x 6 float32) without writing it. */
+dsmc_status dsmc_get_movie_frame(dsmc_handle *h, float *rv6);
 
 /* --- the step: System::step (system.cpp:22-47) ------------------------- */
 dsmc_status dsmc_step(dsmc_handle *h, int nsteps);

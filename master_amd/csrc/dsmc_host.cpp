@@ -100,6 +100,14 @@ struct dsmc_handle {
     unsigned long long *d_st_cnt = nullptr, *d_st_vxh = nullptr, *d_st_n = nullptr;
     double *d_st_vz = nullptr, *d_st_ez = nullptr, *d_st_ke = nullptr;
     long long *d_st_per = nullptr;
+    // movie (DSMC_IO::save_state_to_movie_file): every = 0 is off
+    int movie_every = 0;
+    uint32_t movie_count = 0;
+    FILE *movie_file = nullptr;
+    int64_t movie_last = -1;        // step count of the last frame written
+    uint32_t *d_ids[2] = {};        // particle ids carried by the sort while a movie is set
+    float *d_frame = nullptr;       // movie_count x 6, kept between frames
+    double *d_movie_h = nullptr;    // voxel edge per member and axis
 };
 
 namespace {
@@ -323,6 +331,7 @@ void free_particles(dsmc_handle *h) {
     }
     dfree(h->d_mv_idx); dfree(h->d_mv_cell); dfree(h->d_bucket); dfree(h->d_slow_idx);
     dfree(h->d_rec_perm); dfree(h->d_rec_nhit); dfree(h->d_rec_last);
+    dfree(h->d_ids[0]); dfree(h->d_ids[1]);
 }
 
 void drop_graphs(dsmc_handle *h) {
@@ -450,6 +459,8 @@ GatherArgs gather_args(dsmc_handle *h, bool collide, bool sample) {
     g.mv_cell = h->d_mv_cell;
     g.slow_idx = h->d_slow_idx;
     g.slow_count = h->d_slow_cnt;
+    g.ids = h->d_ids[a];
+    g.ids2 = h->d_ids[b];
     return g;
 }
 
@@ -509,8 +520,23 @@ BucketArgs bucket_args(dsmc_handle *h) {
 }
 
 // Sort a freshly written, unsorted buffer into cell order (stable by index).
+// Particle ids 0..N-1 in the current buffer's order, carried by every later
+// sort (gather_args): the movie's tracked particles are ids below its count.
+dsmc_status track_ids(dsmc_handle *h) {
+    for (int b = 0; b < 2; b++)
+        if (!h->d_ids[b]) HIPCHK(h, dalloc(&h->d_ids[b], h->N));
+    HIPCHK(h, launch_iota_u32(h->d_ids[h->cur], (size_t)h->N, h->stream));
+    return DSMC_OK;
+}
+
 dsmc_status sort_unsorted(dsmc_handle *h, const std::vector<uint32_t> &part_member) {
     hipStream_t s = h->stream;
+    if (h->movie_every) {
+        // a movie set before the particles exist tracks the seeded (or
+        // loaded) order, as the reference's molecule storage order
+        dsmc_status st = track_ids(h);
+        if (st != DSMC_OK) return st;
+    }
     uint32_t *d_pm = nullptr;
     HIPCHK(h, dalloc(&d_pm, h->N));
     HIPCHK(h, hipMemcpyAsync(d_pm, part_member.data(), h->N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
@@ -581,7 +607,9 @@ void free_stats(dsmc_handle *h) {
 // Whether the fused pass may move for the next step: every phase on, one
 // fused pass, nothing that reads the state between steps.
 bool move_ahead_ok(const dsmc_handle *h) {
-    if (!h->move_ahead || !h->fused || h->phases != 7 || h->record || h->totals || h->stats_interval) return false;
+    if (!h->move_ahead || !h->fused || h->phases != 7 || h->record || h->totals || h->stats_interval ||
+        h->movie_every)
+        return false;
     for (auto &m : h->mem)
         if (m.dev.wall_model == DSMC_WALL_CERCIGNANI_LAMPIS) return false;
     return true;
@@ -799,6 +827,90 @@ int64_t reference_count(const MemberHost &m) {
 }  // namespace
 
 // ====================================================================== C ABI
+namespace {
+
+// nsteps steps: captured graphs where they fit, else eager chains.
+dsmc_status run_steps(dsmc_handle *h, int nsteps) {
+    int done = 0;
+    // eager steps chain the next step's move into the fused pass; a chain
+    // (and every captured graph) starts with k_move and ends without the
+    // next move, so the state between dsmc_step calls is a whole step's
+    const bool ahead = move_ahead_ok(h);
+    bool moved = false;
+    while (done < nsteps) {
+        bool sample = h->sample_every > 0 && ((h->steps + 1) % h->sample_every) == 0;
+        bool graph_ok = h->use_graph && !h->timing && nsteps - done >= h->graph_steps &&
+                        (h->sample_every == 0 || h->sample_every == 1);
+        if (graph_ok) {
+            int par = h->cur;
+            if (!h->graph[par] || h->graph_sample[par] != sample) {
+                if (h->graph[par]) {
+                    (void)hipGraphExecDestroy(h->graph[par]);
+                    (void)hipGraphDestroy(h->graph_src[par]);
+                    h->graph[par] = nullptr;
+                    h->graph_src[par] = nullptr;
+                }
+                dsmc_status st = capture_graph(h, par, sample);
+                if (st != DSMC_OK) return st;
+            }
+            HIPCHK(h, hipGraphLaunch(h->graph[par], h->stream));
+            for (int k = 0; k < h->graph_steps; k++) {
+                h->steps++;
+                h->t += h->mem[0].s.dt;
+                if (sample) h->nsamples++;
+            }
+            done += h->graph_steps;
+            continue;
+        }
+        const bool next = ahead && done + 1 < nsteps;
+        dsmc_status st = enqueue_step(h, sample, moved, next);
+        if (st != DSMC_OK) return st;
+        moved = next;
+        h->steps++;
+        h->t += h->mem[0].s.dt;
+        if (sample) h->nsamples++;
+        done++;
+    }
+    return DSMC_OK;
+}
+
+// The tracked particles' rows into the frame buffer (rows of ids at or
+// above N keep their last value: zero, as the reference's zeroed buffer).
+dsmc_status movie_frame(dsmc_handle *h) {
+    MovieArgs a;
+    const int c = h->cur;
+    a.ox = h->ox[c]; a.oy = h->oy[c]; a.oz = h->oz[c]; a.vx = h->vx[c]; a.vy = h->vy[c]; a.vz = h->vz[c];
+    a.vox = h->vox[c];
+    a.ids = h->d_ids[c];
+    a.start = h->d_start[h->scur];
+    a.members = h->d_members;
+    a.h = h->d_movie_h;
+    a.frame = h->d_frame;
+    a.n = (uint32_t)h->N;
+    a.count = h->movie_count;
+    a.nmembers = (uint32_t)h->mem.size();
+    HIPCHK(h, launch_movie_frame(a, h->stream));
+    return DSMC_OK;
+}
+
+// DSMC_IO::save_state_to_movie_file (dsmc_io.cpp:50-80): int count, then
+// count x (r, v) as float32.
+dsmc_status write_movie_frame(dsmc_handle *h) {
+    dsmc_status st = movie_frame(h);
+    if (st != DSMC_OK) return st;
+    std::vector<float> f((size_t)h->movie_count * 6);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (!f.empty()) HIPCHK(h, hipMemcpy(f.data(), h->d_frame, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const int32_t n = (int32_t)h->movie_count;
+    if (std::fwrite(&n, sizeof n, 1, h->movie_file) != 1 ||
+        std::fwrite(f.data(), sizeof(float), f.size(), h->movie_file) != f.size() || std::fflush(h->movie_file))
+        return fail(h, DSMC_ERR_IO, "movie frame write failed");
+    h->movie_last = h->steps;
+    return DSMC_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *dsmc_last_error(const dsmc_handle *h) { return h ? h->err.c_str() : "null handle"; }
@@ -843,6 +955,8 @@ void dsmc_destroy(dsmc_handle *h) {
     dfree(h->d_slow_cnt);
     dfree(h->d_step); dfree(h->d_mvcount); dfree(h->d_cnt); dfree(h->d_partial);
     free_stats(h);
+    if (h->movie_file) std::fclose(h->movie_file);
+    dfree(h->d_frame); dfree(h->d_movie_h);
     for (auto &t : h->timers)
         for (auto &e : t.ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1078,51 +1192,71 @@ dsmc_status dsmc_load_state_file(dsmc_handle *h, const char *path) {
     return dsmc_load_state(h, rv.data(), n);
 }
 
+
 dsmc_status dsmc_step(dsmc_handle *h, int nsteps) {
     if (!h || nsteps < 0) return DSMC_ERR_INVALID_ARGUMENT;
     if (!h->initialized) return fail(h, DSMC_ERR_STATE, "step before initialize (system.cpp:22)");
     if (h->timing)
         for (auto &t : h->timers) t.used = 0;
+    if (!h->movie_every) return run_steps(h, nsteps);
+    // main.cpp:57-59: a frame before every step whose count is a multiple
+    // of movie_every_n_frame, then the step
     int done = 0;
-    // eager steps chain the next step's move into the fused pass; a chain
-    // (and every captured graph) starts with k_move and ends without the
-    // next move, so the state between dsmc_step calls is a whole step's
-    const bool ahead = move_ahead_ok(h);
-    bool moved = false;
     while (done < nsteps) {
-        bool sample = h->sample_every > 0 && ((h->steps + 1) % h->sample_every) == 0;
-        bool graph_ok = h->use_graph && !h->timing && nsteps - done >= h->graph_steps &&
-                        (h->sample_every == 0 || h->sample_every == 1);
-        if (graph_ok) {
-            int par = h->cur;
-            if (!h->graph[par] || h->graph_sample[par] != sample) {
-                if (h->graph[par]) {
-                    (void)hipGraphExecDestroy(h->graph[par]);
-                    (void)hipGraphDestroy(h->graph_src[par]);
-                    h->graph[par] = nullptr;
-                    h->graph_src[par] = nullptr;
-                }
-                dsmc_status st = capture_graph(h, par, sample);
-                if (st != DSMC_OK) return st;
-            }
-            HIPCHK(h, hipGraphLaunch(h->graph[par], h->stream));
-            for (int k = 0; k < h->graph_steps; k++) {
-                h->steps++;
-                h->t += h->mem[0].s.dt;
-                if (sample) h->nsamples++;
-            }
-            done += h->graph_steps;
-            continue;
+        const int64_t every = h->movie_every;
+        if (h->steps % every == 0 && h->movie_last != h->steps) {
+            dsmc_status st = write_movie_frame(h);
+            if (st != DSMC_OK) return st;
         }
-        const bool next = ahead && done + 1 < nsteps;
-        dsmc_status st = enqueue_step(h, sample, moved, next);
+        int64_t seg = every - h->steps % every;
+        if (seg > nsteps - done) seg = nsteps - done;
+        dsmc_status st = run_steps(h, (int)seg);
         if (st != DSMC_OK) return st;
-        moved = next;
-        h->steps++;
-        h->t += h->mem[0].s.dt;
-        if (sample) h->nsamples++;
-        done++;
+        done += (int)seg;
     }
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_set_movie(dsmc_handle *h, const char *path, int every_n_steps, int molecules) {
+    if (!h) return DSMC_ERR_INVALID_ARGUMENT;
+    const bool on = path && every_n_steps > 0;
+    if (on && molecules < 0) return DSMC_ERR_INVALID_ARGUMENT;
+    if (h->movie_file) std::fclose(h->movie_file);
+    h->movie_file = nullptr;
+    h->movie_every = 0;
+    h->movie_count = 0;
+    h->movie_last = -1;
+    dfree(h->d_frame); dfree(h->d_movie_h);
+    dfree(h->d_ids[0]); dfree(h->d_ids[1]);
+    drop_graphs(h);    // the captured sorts carry ids or not
+    if (!on) return DSMC_OK;
+    h->movie_file = std::fopen(path, "wb");
+    if (!h->movie_file) return fail(h, DSMC_ERR_IO, std::string("cannot open movie file ") + path);
+    h->movie_count = (uint32_t)molecules;
+    HIPCHK(h, dalloc(&h->d_frame, (size_t)h->movie_count * 6));
+    HIPCHK(h, hipMemset(h->d_frame, 0, std::max<size_t>((size_t)h->movie_count * 6, 1) * sizeof(float)));
+    std::vector<double> hh(3 * h->mem.size());
+    for (size_t mi = 0; mi < h->mem.size(); mi++) {
+        const MemberHost &m = h->mem[mi];
+        const double L[3] = {m.s.Lx, m.s.Ly, m.s.Lz};
+        for (int a = 0; a < 3; a++) hh[3 * mi + a] = L[a] / m.dev.n[a];
+    }
+    HIPCHK(h, dalloc(&h->d_movie_h, hh.size()));
+    HIPCHK(h, hipMemcpy(h->d_movie_h, hh.data(), hh.size() * sizeof(double), hipMemcpyHostToDevice));
+    h->movie_every = every_n_steps;
+    if (h->initialized) return track_ids(h);
+    return DSMC_OK;
+}
+
+dsmc_status dsmc_get_movie_frame(dsmc_handle *h, float *rv6) {
+    if (!h || (!rv6 && h->movie_count > 0)) return DSMC_ERR_INVALID_ARGUMENT;
+    if (!h->movie_every) return fail(h, DSMC_ERR_STATE, "no movie set (dsmc_set_movie)");
+    if (!h->initialized) return fail(h, DSMC_ERR_STATE, "not initialized");
+    dsmc_status st = movie_frame(h);
+    if (st != DSMC_OK) return st;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->movie_count)
+        HIPCHK(h, hipMemcpy(rv6, h->d_frame, (size_t)h->movie_count * 6 * sizeof(float), hipMemcpyDeviceToHost));
     return DSMC_OK;
 }
 

@@ -136,6 +136,34 @@ extern "C" dsmc_status dsmc_settings_from_ini(const char *path, dsmc_settings *o
     return DSMC_OK;
 }
 
+extern "C" dsmc_status dsmc_run_settings_from_ini(const char *path, dsmc_run_settings *out) {
+    if (!path || !out) return DSMC_ERR_INVALID_ARGUMENT;
+    std::vector<std::pair<std::string, std::string>> items;
+    if (!parse_ini(path, items)) return DSMC_ERR_IO;
+    dsmc_run_settings r;
+    std::memset(&r, 0, sizeof r);
+    // settings.cpp:6-14: every key is required (CIniFile throws without it)
+    auto geti = [&](const char *k, int &v) -> bool {
+        const std::string *s = find(items, k);
+        if (!s) return false;
+        v = (int)std::strtod(s->c_str(), nullptr);
+        return true;
+    };
+    auto getb = [&](const char *k, int &v) -> bool {
+        const std::string *s = find(items, k);
+        if (!s) return false;
+        v = *s == "true";
+        return true;
+    };
+    if (!getb("load_previous_state", r.load_previous_state) || !getb("create_movie", r.create_movie) ||
+        !geti("statistics_interval", r.statistics_interval) ||
+        !geti("movie_every_n_frame", r.movie_every_n_frame) || !geti("timesteps", r.timesteps) ||
+        !geti("movie_molecules", r.movie_molecules) || !geti("sampling_bins", r.sampling_bins))
+        return DSMC_ERR_CONFIG;
+    *out = r;
+    return DSMC_OK;
+}
+
 // ------------------------------------------------------------------ world file
 extern "C" dsmc_status dsmc_world_read(const char *path, dsmc_world *out) {
     if (!path || !out) return DSMC_ERR_INVALID_ARGUMENT;

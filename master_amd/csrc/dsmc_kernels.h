@@ -86,6 +86,10 @@ struct GatherArgs {
     const float *frames;
     uint32_t *leave, *arrive, *mv_idx, *mv_cell;
     uint32_t *slow_idx, *slow_count;    // wall-path particles per cell (new ranges), for k_move_slow
+    // particle ids carried through the sort while a movie tracks particles
+    // (null otherwise; never with the move fused in)
+    const uint32_t *ids;
+    uint32_t *ids2;
 };
 
 struct CollideArgs {
@@ -164,6 +168,21 @@ struct StatsArgs {
     unsigned long long *nsamples;
 };
 hipError_t launch_stats(const StatsArgs &a, hipStream_t s);
+
+// One movie frame (DSMC_IO::save_state_to_movie_file, dsmc_io.cpp:50-80):
+// the tracked particles (id < count) write r (absolute, as dsmc_get_state
+// computes it in FP64) and v as float32 to row id of the frame.
+struct MovieArgs {
+    const float *ox, *oy, *oz, *vx, *vy, *vz;
+    const uint32_t *vox, *ids;
+    const uint32_t *start;      // cell ranges of the current order
+    const Member *members;
+    const double *h;            // voxel edge per member and axis
+    float *frame;               // count x 6
+    uint32_t n, count, nmembers;
+};
+hipError_t launch_movie_frame(const MovieArgs &a, hipStream_t s);
+hipError_t launch_iota_u32(uint32_t *p, size_t n, hipStream_t s);
 
 // move_ahead: the pass also moves the cells' particles for the next step
 // (then launch_move_slow with step_offset 1 finishes that move).

@@ -586,6 +586,7 @@ __global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
                 a.vx2[d] = vx[q]; a.vy2[d] = vy[q]; a.vz2[d] = vz[q];
                 a.vox2[d] = vp[q];
                 if (a.rec_perm) a.rec_perm[d] = (int32_t)p;
+                if (a.ids) a.ids2[d] = a.ids[p];
             }
             run += (uint32_t)__popcll(sm);
         }
@@ -614,6 +615,7 @@ __global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
             a.vx2[d] = a.vx[q]; a.vy2[d] = a.vy[q]; a.vz2[d] = a.vz[q];
             a.vox2[d] = a.vox[q];
             if (a.rec_perm) a.rec_perm[d] = (int32_t)q;
+            if (a.ids) a.ids2[d] = a.ids[q];
         }
     }
 }
@@ -1091,6 +1093,7 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
                 if constexpr (!MOVE) {
                     a.ox2[d] = P.ox[q]; a.oy2[d] = P.oy[q]; a.oz2[d] = P.oz[q];
                     a.vox2[d] = P.vp[q];
+                    if (a.ids) a.ids2[d] = a.ids[p];
                 }
                 if (in_lds) { lx[L] = P.vx[q]; ly[L] = P.vy[q]; lz[L] = P.vz[q]; }
                 else { a.vx2[d] = P.vx[q]; a.vy2[d] = P.vy[q]; a.vz2[d] = P.vz[q]; }
@@ -1106,6 +1109,7 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
         if (in_lds) { lx[aL] = avx; ly[aL] = avy; lz[aL] = avz; }
         else { a.vx2[d] = avx; a.vy2[d] = avy; a.vz2[d] = avz; }
         if (a.rec_perm) a.rec_perm[d] = (int32_t)key0;
+        if (!MOVE && a.ids) a.ids2[d] = a.ids[key0];
     }
     // more than 64 arrivals: ranks from the keys in LDS, or in the bucket
     for (uint32_t base = 0; !few && base < nb; base += WAVE) {
@@ -1128,6 +1132,7 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
             if (in_lds) { lx[L] = x; ly[L] = y; lz[L] = z; }
             else { a.vx2[d] = x; a.vy2[d] = y; a.vz2[d] = z; }
             if (a.rec_perm) a.rec_perm[d] = (int32_t)q;
+            if (!MOVE && a.ids) a.ids2[d] = a.ids[q];
         }
     }
     if (in_lds) {
@@ -1595,6 +1600,37 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
         p[i] = v;
 }
 
+__global__ void k_iota_u32(uint32_t *p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
+// Movie frame: each tracked particle (id < count) writes its row.  The
+// member holding particle p is the last whose first particle (the start of
+// its first cell) is at most p.  r = (voxel + offset) * h in FP64, as
+// dsmc_get_state, then float32 as the reference's frame buffer holds it.
+__global__ __launch_bounds__(256) void k_movie_frame(MovieArgs a) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += gridDim.x * blockDim.x) {
+        const uint32_t id = a.ids[p];
+        if (id >= a.count) continue;
+        uint32_t lo = 0, hi = a.nmembers;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.start[a.members[mid].cell0] <= p) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t v = a.vox[p];
+        const double *h = a.h + 3 * (size_t)lo;
+        float *f = a.frame + (size_t)id * 6;
+        f[0] = (float)(((double)(v >> 20) + (double)a.ox[p]) * h[0]);
+        f[1] = (float)(((double)((v >> 10) & 1023u) + (double)a.oy[p]) * h[1]);
+        f[2] = (float)(((double)(v & 1023u) + (double)a.oz[p]) * h[2]);
+        f[3] = a.vx[p];
+        f[4] = a.vy[p];
+        f[5] = a.vz[p];
+    }
+}
+
 // mv_count[0]: movers of an unsorted buffer; mv_count[1]: the collide
 // kernel's work queue.
 __global__ void k_begin_step(uint32_t *step, uint32_t *mv_count) {
@@ -1874,6 +1910,24 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
     size_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_iota_u32(uint32_t *p, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_iota_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_movie_frame(const MovieArgs &a, hipStream_t s) {
+    if (a.n == 0 || a.count == 0) return hipSuccess;
+    uint32_t blocks = (a.n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_movie_frame, dim3(blocks), dim3(256), 0, s, a);
     LAUNCH_CHECK();
     return hipSuccess;
 }

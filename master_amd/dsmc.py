@@ -59,6 +59,13 @@ class Counters(ctypes.Structure):
                 ("overflow_cells", ctypes.c_uint64)]
 
 
+class RunSettings(ctypes.Structure):
+    """dsmc_run_settings: the run and output keys of dsmc.ini (settings.cpp:6-14)."""
+    _fields_ = [("timesteps", ctypes.c_int), ("load_previous_state", ctypes.c_int), ("create_movie", ctypes.c_int),
+                ("movie_every_n_frame", ctypes.c_int), ("movie_molecules", ctypes.c_int),
+                ("statistics_interval", ctypes.c_int), ("sampling_bins", ctypes.c_int)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("graph_steps", ctypes.c_int), ("collide_chunk", ctypes.c_int), ("ms_per_step", D),
                 ("ms_per_step_default", D), ("trials", ctypes.c_int)]
@@ -82,7 +89,8 @@ def lib():
                      "dsmc_geometry_packed_spheres", "dsmc_geometry_poly_spheres", "dsmc_world_write",
                      "dsmc_save_state_file", "dsmc_load_state_file", "dsmc_initialize_fields",
                      "dsmc_autotune", "dsmc_set_statistics", "dsmc_get_statistics",
-                     "dsmc_get_statistics_series", "dsmc_write_statistics"):
+                     "dsmc_get_statistics_series", "dsmc_write_statistics", "dsmc_run_settings_from_ini",
+                     "dsmc_set_movie", "dsmc_get_movie_frame"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.dsmc_destroy.restype = None
         _lib.dsmc_world_free.restype = None
@@ -129,6 +137,13 @@ def settings_from_ini(ini: dict) -> Settings:
         s.alpha_n, s.alpha_t = float(ini["alpha_n"]), float(ini["alpha_t"])
     s.max_molecules = int(float(ini.get("max_molecules_per_node", 0)))
     return s
+
+
+def run_settings_from_ini_file(path) -> dict:
+    """The run and output keys of a dsmc.ini (dsmc_run_settings_from_ini)."""
+    r = RunSettings()
+    _check(lib().dsmc_run_settings_from_ini(os.fsencode(str(path)), ctypes.byref(r)), what="run_settings_from_ini")
+    return {name: getattr(r, name) for name, _ in RunSettings._fields_}
 
 
 def settings_from_ini_file(path) -> tuple[Settings, str]:
@@ -336,6 +351,20 @@ class System:
 
     def write_statistics(self, directory):
         self._call("dsmc_write_statistics", os.fsencode(str(directory)))
+
+    # DSMC_IO::save_state_to_movie_file (dsmc_io.cpp:50-80)
+    def set_movie(self, path, every_n_steps=1, molecules=0):
+        """Frames to `path` before each step whose count is a multiple of
+        every_n_steps (path None: off)."""
+        self._movie_molecules = int(molecules) if path is not None else 0
+        self._call("dsmc_set_movie", None if path is None else os.fsencode(str(path)), ctypes.c_int(every_n_steps),
+                   ctypes.c_int(molecules))
+
+    def movie_frame(self):
+        """The current state's frame: (molecules, 6) float32 r, v."""
+        f = np.zeros((getattr(self, "_movie_molecules", 0), 6), np.float32)
+        self._call("dsmc_get_movie_frame", _ptr(f))
+        return f
 
     def totals(self):
         t = np.zeros(4)

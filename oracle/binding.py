@@ -64,6 +64,7 @@ def ref():
         _ref.ref_sys_num.restype = L
         _ref.ref_sys_log_size.restype = L
         _ref.ref_cl_normal_samples.restype = L
+        _ref.ref_sys_movie_frame.restype = ctypes.c_int
         for f in ("ref_sys_num", "ref_sys_get_state", "ref_sys_set_state", "ref_sys_move_all",
                   "ref_sys_step", "ref_sys_counters", "ref_sys_log_size", "ref_sys_log",
                   "ref_sys_log_clear", "ref_sys_num_active", "ref_sys_cells", "ref_sys_cell_volumes"):
@@ -378,6 +379,7 @@ class RefSystem:
             workdir = self.tmp.name
         self.workdir = workdir
         os.makedirs(os.path.join(workdir, "statistics", "time"), exist_ok=True)
+        os.makedirs(os.path.join(workdir, "movie_files"), exist_ok=True)
         write_world_file(os.path.join(workdir, "world"), world)
         ini = dict(ini)
         ini["world"] = "world"
@@ -418,6 +420,15 @@ class RefSystem:
         self.r.ref_sys_stats_finalize(P(self.h))
         d = os.path.join(self.workdir, "statistics")
         return {f: open(os.path.join(d, f)).read() for f in os.listdir(d) if f.endswith(".txt")}
+
+    def movie_frame(self):
+        """The reference's movie writer for the current step (a frame when the
+        step count is a multiple of movie_every_n_frame and create_movie is
+        true); returns the movie file's bytes so far."""
+        if self.r.ref_sys_movie_frame(P(self.h), self.workdir.encode()) != 0:
+            raise RuntimeError("reference movie writer: work directory")
+        path = os.path.join(self.workdir, "movie_files", "movie0000.bin")
+        return open(path, "rb").read() if os.path.exists(path) else b""
 
     def counters(self):
         o = np.zeros(8)

@@ -120,6 +120,18 @@ void ref_sys_stats_finalize(void *hp) {
     h->sys->io->finalize();
 }
 
+// DSMC_IO::save_state_to_movie_file (dsmc_io.cpp:50-80) as main.cpp:57-59
+// calls it before each step: in the work directory, so the frame goes to
+// movie_files/movie0000.bin (flushed for the caller to read).
+int ref_sys_movie_frame(void *hp, const char *workdir) {
+    Handle *h = (Handle *)hp;
+    char old[4096];
+    if (!getcwd(old, sizeof old) || chdir(workdir) != 0) return -1;
+    h->sys->io->save_state_to_movie_file();
+    if (h->sys->io->movie_file_open) h->sys->io->movie_file->flush();
+    return chdir(old) == 0 ? 0 : -1;
+}
+
 // out: steps, t, collisions, wall collisions, count_periodic[3], log length
 void ref_sys_counters(void *hp, double *out) {
     Handle *h = (Handle *)hp;
