@@ -171,6 +171,12 @@ dsmc_status dsmc_get_movie_frame(dsmc_handle *h, float *rv6);
 dsmc_status dsmc_step(dsmc_handle *h, int nsteps);
 dsmc_status dsmc_synchronize(dsmc_handle *h);
 dsmc_status dsmc_get_counters(dsmc_handle *h, dsmc_counters *c);
+/* Resume (main.cpp:37-55, the Tocontinue file): set System::t, steps,
+ * collisions and the wall collider's num_collisions.  The step count keys
+ * the step's random draws, so the run continues the counter stream.  Call
+ * before dsmc_set_statistics, which takes the run's time origin (t0). */
+dsmc_status dsmc_set_counters(dsmc_handle *h, int64_t steps, double t, uint64_t collisions,
+                              uint64_t wall_collisions);
 /* The reference chooses the collider from surface_interaction_model
  * (system.cpp:343-353); the Cercignani-Lampis model uses the settings'
  * alpha_n and alpha_t. */

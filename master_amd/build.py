@@ -2,6 +2,7 @@
 
 * master_amd/libdsmc.so   the product: HIP kernels for gfx950 + host System
                           + C ABI (include/dsmc.h), compiled with hipcc
+* master_amd/dsmc_run     the reference's main program over the C ABI, g++
 * oracle/liboracle.so     the CPU oracle (test infrastructure), gcc
 * oracle/_ref/...         the reference harness, only where the reference
                           sources exist (oracle/Makefile.ref)
@@ -56,6 +57,17 @@ def build_lib():
     return out
 
 
+def build_driver():
+    """master_amd/dsmc_run: the reference's main program over the C ABI
+    (host C++ only, linked against libdsmc.so next to it)."""
+    src = os.path.join(CSRC, "dsmc_run.cpp")
+    out = os.path.join(ROOT, "master_amd", "dsmc_run")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", src, "-o", out + ".tmp",
+          "-L" + os.path.join(ROOT, "master_amd"), "-ldsmc", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle():
     src = os.path.join(ROOT, "oracle", "dsmc_oracle.c")
     out = os.path.join(ROOT, "oracle", "liboracle.so")
@@ -75,7 +87,8 @@ def build_ref(ref_root="/root/reference"):
 
 
 def build_all():
-    return {"lib": build_lib(), "oracle": build_oracle(), "ref": build_ref()}
+    lib = build_lib()
+    return {"lib": lib, "driver": build_driver(), "oracle": build_oracle(), "ref": build_ref()}
 
 
 if __name__ == "__main__":

@@ -1266,6 +1266,21 @@ dsmc_status dsmc_synchronize(dsmc_handle *h) {
     return DSMC_OK;
 }
 
+dsmc_status dsmc_set_counters(dsmc_handle *h, int64_t steps, double t, uint64_t collisions,
+                              uint64_t wall_collisions) {
+    if (!h || steps < 0 || steps > 0xffffffffll) return DSMC_ERR_INVALID_ARGUMENT;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const uint32_t st = (uint32_t)steps;
+    HIPCHK(h, hipMemcpy(h->d_step, &st, sizeof st, hipMemcpyHostToDevice));
+    const unsigned long long c = collisions, w = wall_collisions;
+    HIPCHK(h, hipMemcpy(h->d_cnt + CNT_COLL, &c, sizeof c, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_cnt + CNT_WALL, &w, sizeof w, hipMemcpyHostToDevice));
+    h->steps = steps;
+    h->t = t;
+    h->movie_last = -1;
+    return DSMC_OK;
+}
+
 dsmc_status dsmc_get_counters(dsmc_handle *h, dsmc_counters *c) {
     if (!h || !c) return DSMC_ERR_INVALID_ARGUMENT;
     unsigned long long v[CNT_NUM];

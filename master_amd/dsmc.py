@@ -90,7 +90,7 @@ def lib():
                      "dsmc_save_state_file", "dsmc_load_state_file", "dsmc_initialize_fields",
                      "dsmc_autotune", "dsmc_set_statistics", "dsmc_get_statistics",
                      "dsmc_get_statistics_series", "dsmc_write_statistics", "dsmc_run_settings_from_ini",
-                     "dsmc_set_movie", "dsmc_get_movie_frame"):
+                     "dsmc_set_movie", "dsmc_get_movie_frame", "dsmc_set_counters"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.dsmc_destroy.restype = None
         _lib.dsmc_world_free.restype = None
@@ -309,6 +309,11 @@ class System:
         c = Counters()
         self._call("dsmc_get_counters", ctypes.byref(c))
         return {f: (list(getattr(c, f)) if f == "count_periodic" else getattr(c, f)) for f, _ in Counters._fields_}
+
+    def set_counters(self, steps, t, collisions=0, wall_collisions=0):
+        """Resume counters (main.cpp:37-55, Tocontinue)."""
+        self._call("dsmc_set_counters", I64(steps), D(t), ctypes.c_uint64(collisions),
+                   ctypes.c_uint64(wall_collisions))
 
     def set_option(self, opt, value):
         self._call("dsmc_set_option", ctypes.c_int(opt), I64(int(value)))
