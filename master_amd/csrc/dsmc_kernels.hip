@@ -756,6 +756,28 @@ __device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint3
     acc_out = acc;
 }
 
+// Per-cell FP64 moments, per lane: sums of v and v^2 (the count is the
+// cell's n, set by moments_commit: a sum of 1.0 per particle is exact).
+__device__ __forceinline__ void moments_add(double *mm, double dx, double dy, double dz) {
+    mm[1] += dx; mm[2] += dy; mm[3] += dz;
+    mm[4] = __builtin_fma(dx, dx, mm[4]);
+    mm[5] = __builtin_fma(dy, dy, mm[5]);
+    mm[6] = __builtin_fma(dz, dz, mm[6]);
+}
+
+// The wave's sums of a cell's moments, added to the accumulators with one
+// FP64 atomic per component (lanes 0-6).
+__device__ __forceinline__ void moments_commit(double *mm, uint32_t n, double *moments, uint32_t ncells,
+                                               uint32_t c) {
+    const uint32_t lane = threadIdx.x;
+#pragma unroll
+    for (int q = 1; q < 7; q++) mm[q] = wave_sum(mm[q]);
+    double mine = (double)n;
+#pragma unroll
+    for (int q = 1; q < 7; q++) mine = lane == (uint32_t)q ? mm[q] : mine;
+    if (lane < 7) unsafeAtomicAdd(moments + (size_t)lane * ncells + c, mine);
+}
+
 extern __shared__ float dyn_lds[];
 
 // Per-cell values the collide loop needs before it can start.
@@ -924,24 +946,17 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
                     const float x = lx[i], y = ly[i], z = lz[i];
                     if (a.do_collide) { a.vx[s + i] = x; a.vy[s + i] = y; a.vz[s + i] = z; }
                     const double dx = x, dy = y, dz = z;
-                    mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
-                    mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+                    moments_add(mm, dx, dy, dz);
                 }
             } else {
                 for (uint32_t i = lane; i < n; i += WAVE) {
                     const double dx = GlobalStore::ldf(a.vx + s + i), dy = GlobalStore::ldf(a.vy + s + i),
                                  dz = GlobalStore::ldf(a.vz + s + i);
-                    mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
-                    mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+                    moments_add(mm, dx, dy, dz);
                 }
             }
             if (a.do_sample) {
-#pragma unroll
-                for (int q = 0; q < 7; q++) mm[q] = wave_sum(mm[q]);
-                double mine = mm[0];
-#pragma unroll
-                for (int q = 1; q < 7; q++) mine = lane == (uint32_t)q ? mm[q] : mine;
-                if (lane < 7) unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + c, mine);
+                moments_commit(mm, n, a.moments, a.ncells, c);
             }
         }
         m0 = m1;
@@ -1189,24 +1204,17 @@ __device__ __forceinline__ void gc_write_back(const GatherArgs &a, uint32_t c, c
             const float x = lx[i], y = ly[i], z = lz[i];
             a.vx2[ns + i] = x; a.vy2[ns + i] = y; a.vz2[ns + i] = z;
             const double dx = x, dy = y, dz = z;
-            mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
-            mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+            moments_add(mm, dx, dy, dz);
         }
     } else {
         for (uint32_t i = lane; i < n; i += WAVE) {
             const double dx = GlobalStore::ldf(a.vx2 + ns + i), dy = GlobalStore::ldf(a.vy2 + ns + i),
                          dz = GlobalStore::ldf(a.vz2 + ns + i);
-            mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
-            mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+            moments_add(mm, dx, dy, dz);
         }
     }
     if (a.do_sample && active) {
-#pragma unroll
-        for (int q = 0; q < 7; q++) mm[q] = wave_sum(mm[q]);
-        double mine = mm[0];
-#pragma unroll
-        for (int q = 1; q < 7; q++) mine = lane == (uint32_t)q ? mm[q] : mine;
-        if (lane < 7) unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + c, mine);
+        moments_commit(mm, n, a.moments, a.ncells, c);
     }
 }
 
@@ -1362,15 +1370,9 @@ __device__ __forceinline__ void gc_write_back_move(const GatherArgs &a, uint32_t
                 dx = GlobalStore::ldf(a.vx2 + ns + i); dy = GlobalStore::ldf(a.vy2 + ns + i);
                 dz = GlobalStore::ldf(a.vz2 + ns + i);
             }
-            mm[0] += 1.0; mm[1] += dx; mm[2] += dy; mm[3] += dz;
-            mm[4] += dx * dx; mm[5] += dy * dy; mm[6] += dz * dz;
+            moments_add(mm, dx, dy, dz);
         }
-#pragma unroll
-        for (int q = 0; q < 7; q++) mm[q] = wave_sum(mm[q]);
-        double mine = mm[0];
-#pragma unroll
-        for (int q = 1; q < 7; q++) mine = lane == (uint32_t)q ? mm[q] : mine;
-        if (lane < 7) unsafeAtomicAdd(a.moments + (size_t)lane * a.ncells + c, mine);
+        moments_commit(mm, n, a.moments, a.ncells, c);
     }
     int32_t per[3] = {0, 0, 0};
     uint32_t leavers = 0, nslow = 0;
