@@ -607,8 +607,7 @@ void free_stats(dsmc_handle *h) {
 // Whether the fused pass may move for the next step: every phase on, one
 // fused pass, nothing that reads the state between steps.
 bool move_ahead_ok(const dsmc_handle *h) {
-    if (!h->move_ahead || !h->fused || h->phases != 7 || h->record || h->totals || h->stats_interval ||
-        h->movie_every)
+    if (!h->move_ahead || !h->fused || h->phases != 7 || h->record || h->totals || h->movie_every)
         return false;
     for (auto &m : h->mem)
         if (m.dev.wall_model == DSMC_WALL_CERCIGNANI_LAMPIS) return false;
@@ -729,10 +728,6 @@ dsmc_status enqueue_step(dsmc_handle *h, bool sample, bool move_prev = false, bo
         int b = h->cur;
         HIPCHK(h, launch_totals_partial(h->vx[b], h->vy[b], h->vz[b], (uint32_t)h->N,
                                         h->d_partial + TOT_BLOCKS * 4, TOT_BLOCKS, s));
-    }
-    if (h->stats_interval) {
-        Scope sc(h, "k_stats");
-        HIPCHK(h, launch_stats(stats_args(h), s));
     }
     return DSMC_OK;
 }
@@ -1198,21 +1193,28 @@ dsmc_status dsmc_step(dsmc_handle *h, int nsteps) {
     if (!h->initialized) return fail(h, DSMC_ERR_STATE, "step before initialize (system.cpp:22)");
     if (h->timing)
         for (auto &t : h->timers) t.used = 0;
-    if (!h->movie_every) return run_steps(h, nsteps);
-    // main.cpp:57-59: a frame before every step whose count is a multiple
-    // of movie_every_n_frame, then the step
+    // The steps run in segments that end where the host has work between
+    // two steps: a movie frame before a step whose count is a multiple of
+    // movie_every_n_frame (main.cpp:57-59), a statistics sample after one
+    // whose count is a multiple of statistics_interval
+    // (statisticalproperty.cpp:22).  Inside a segment the steps chain as
+    // without them (graphs, move-ahead); each segment ends with a whole step.
     int done = 0;
     while (done < nsteps) {
-        const int64_t every = h->movie_every;
-        if (h->steps % every == 0 && h->movie_last != h->steps) {
+        if (h->movie_every && h->steps % h->movie_every == 0 && h->movie_last != h->steps) {
             dsmc_status st = write_movie_frame(h);
             if (st != DSMC_OK) return st;
         }
-        int64_t seg = every - h->steps % every;
-        if (seg > nsteps - done) seg = nsteps - done;
+        int64_t seg = nsteps - done;
+        if (h->movie_every) seg = std::min<int64_t>(seg, h->movie_every - h->steps % h->movie_every);
+        if (h->stats_interval) seg = std::min<int64_t>(seg, h->stats_interval - h->steps % h->stats_interval);
         dsmc_status st = run_steps(h, (int)seg);
         if (st != DSMC_OK) return st;
         done += (int)seg;
+        if (h->stats_interval && h->steps % h->stats_interval == 0) {
+            Scope sc(h, "k_stats");
+            HIPCHK(h, launch_stats(stats_args(h), h->stream));
+        }
     }
     return DSMC_OK;
 }

@@ -77,7 +77,8 @@ def test_restatement_reproduces_the_reference_sampler():
 @pytest.mark.gpu
 def test_device_sampler_and_writer_match_restatement(gpu, tmp_path):
     """Bins counted exactly, FP64 sums to rounding; the written files equal
-    the restatement's on the device's own states."""
+    the restatement's on the device's own states; graphs and chained eager
+    steps sample the same steps and leave the trajectory unchanged."""
     dsmc = gpu
     w = K.world()
     ini = _ini(w, 100_000)
@@ -116,5 +117,16 @@ def test_device_sampler_and_writer_match_restatement(gpu, tmp_path):
     assert d2["samples"] == dev["samples"]
     assert np.array_equal(d2["count"], dev["count"]) and np.array_equal(d2["vx_hist"], dev["vx_hist"])
     assert np.allclose(d2["vz_sum"], dev["vz_sum"], rtol=1e-12, atol=1e-14)
+    # one call of eager steps: they chain with the next step's move fused
+    # in between samples, and the trajectory is the one without statistics
+    g3 = dsmc.System(ini, w)
+    g3.initialize()
+    g3.set_statistics(INTERVAL, BINS, 64)
+    g3.step(STEPS)
+    d3 = g3.statistics()
+    assert d3["samples"] == dev["samples"]
+    assert np.array_equal(d3["count"], dev["count"]) and np.array_equal(d3["vx_hist"], dev["vx_hist"])
+    assert np.array_equal(g3.state(), g.state())
     g.close()
     g2.close()
+    g3.close()
