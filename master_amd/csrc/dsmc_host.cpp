@@ -539,7 +539,8 @@ dsmc_status sort_unsorted(dsmc_handle *h, const std::vector<uint32_t> &part_memb
     }
     uint32_t *d_pm = nullptr;
     HIPCHK(h, dalloc(&d_pm, h->N));
-    HIPCHK(h, hipMemcpyAsync(d_pm, part_member.data(), h->N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (h->N > 0)
+        HIPCHK(h, hipMemcpyAsync(d_pm, part_member.data(), h->N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     HIPCHK(h, launch_fill_u32(h->d_start[h->scur], 0u, h->ncells + 1, s));
     HIPCHK(h, launch_fill_u32(h->d_leave, 0u, h->ncells, s));
     HIPCHK(h, launch_fill_u32(h->d_arrive, 0u, h->ncells, s));
@@ -1104,13 +1105,15 @@ dsmc_status dsmc_load_state(dsmc_handle *h, const double *rv6, int64_t n) {
         vx[p] = ((uint32_t)idx[0] << 20) | ((uint32_t)idx[1] << 10) | (uint32_t)idx[2];
     }
     int c = h->cur;
-    HIPCHK(h, hipMemcpy(h->ox[c], o[0].data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->oy[c], o[1].data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->oz[c], o[2].data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->vx[c], v[0].data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->vy[c], v[1].data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->vz[c], v[2].data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->vox[c], vx.data(), n * 4, hipMemcpyHostToDevice));
+    if (n > 0) {
+        HIPCHK(h, hipMemcpy(h->ox[c], o[0].data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->oy[c], o[1].data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->oz[c], o[2].data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->vx[c], v[0].data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->vy[c], v[1].data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->vz[c], v[2].data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->vox[c], vx.data(), n * 4, hipMemcpyHostToDevice));
+    }
     m.first = 0;
     m.count = n;
     std::vector<uint32_t> pm(n, 0u);
@@ -1130,6 +1133,7 @@ dsmc_status dsmc_get_state(dsmc_handle *h, double *rv6) {
     if (!h->initialized) return fail(h, DSMC_ERR_STATE, "not initialized");
     int64_t n = h->N;
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (n == 0) return DSMC_OK;
     std::vector<float> o[3], v[3];
     std::vector<uint32_t> vx(n);
     for (int a = 0; a < 3; a++) { o[a].resize(n); v[a].resize(n); }

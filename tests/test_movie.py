@@ -106,7 +106,7 @@ def test_rows_track_particles_between_frames(gpu, tmp_path):
     (periodic wrap aside).  Rows of other particles would not."""
     dsmc = gpu
     w = K.world()
-    ini = K.settings(w, 20000)
+    ini = K.settings(w, 20000, gravity=0.0)
     g = dsmc.System(ini, w)
     g.initialize()
     m = 4000
