@@ -527,7 +527,8 @@ static constexpr int GATHER_BATCH = 8;      // chunks of 64 particles per round 
 // batch is in registers during a cell's write-back.
 static constexpr int GC_BATCH_SMALL = 5, GC_BATCH_LARGE = 11;
 // With the next step's move in the pass (gc_write_back_move) the batch holds
-// no positions: 5 (96 VGPRs) and 14 (153 VGPRs).
+// no positions: 5 (96 VGPRs) and 14 (160 VGPRs, with the move's stayers two
+// chunks ahead).
 static constexpr int GC_BATCH_LARGE_MOVE = 14;
 
 __global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
@@ -1287,12 +1288,15 @@ __device__ __forceinline__ void gc_move_one(const GatherArgs &a, uint32_t c, uin
 }
 
 // The move of gc_write_back_move.  Stayers come from the old buffer (which
-// this pass does not write), one chunk ahead, the gather's classification
-// repeated to place them; arrivals, slots [0, n_low) and
+// this pass does not write), two chunks ahead in three register sets whose
+// roles rotate with the unrolled loop (a copy between sets would wait for
+// its load), the gather's classification repeated to place them (DEEP;
+// else one chunk ahead, for the small-batch variant's registers); A holds
+// the first chunk, loaded by the caller.  Arrivals, slots [0, n_low) and
 // [n_low + n_stay, n), from the new buffer, where the gather stored them.
 // IN_LDS: the velocities' source (a merged path would make the compiler wait
 // for every load in flight, the prefetch included, before the first use).
-template <bool IN_LDS>
+template <bool IN_LDS, bool DEEP>
 __device__ __forceinline__ void gc_move_cell(const GatherArgs &a, uint32_t c, uint32_t ns, uint32_t os,
                                              uint32_t no, uint32_t nb, uint32_t n_low, uint32_t n_stay,
                                              const Member &M, const VoxelView &V, const float *lx,
@@ -1300,10 +1304,8 @@ __device__ __forceinline__ void gc_move_cell(const GatherArgs &a, uint32_t c, ui
                                              uint32_t &leavers, uint32_t &nslow) {
     const uint32_t lane = threadIdx.x;
     uint32_t run = 0;
-    for (uint32_t base = 0; base < no; base += WAVE) {
+    auto stayers = [&](uint32_t base, const GcPos &cur) {
         const uint32_t i = base + lane;
-        const GcPos cur = A;
-        A = gc_load_pos(a.ox, a.oy, a.oz, a.vox, os + (i + WAVE < no ? i + WAVE : no - 1));
         bool stay = false;
         if (i < no) {
             int ix, iy, iz;
@@ -1314,6 +1316,30 @@ __device__ __forceinline__ void gc_move_cell(const GatherArgs &a, uint32_t c, ui
         const uint32_t L = stay ? n_low + run + mbcnt(sm) : 0u;
         gc_move_one<IN_LDS>(a, c, ns, L, stay, M, V, lx, ly, lz, cur, per, leavers, nslow);
         run += (uint32_t)__popcll(sm);
+    };
+    auto load = [&](uint32_t base) {
+        const uint32_t i = base + lane;
+        return gc_load_pos(a.ox, a.oy, a.oz, a.vox, os + (i < no ? i : no - 1));
+    };
+    if (!DEEP) {
+        for (uint32_t base = 0; base < no; base += WAVE) {
+            const GcPos cur = A;
+            A = load(base + WAVE);
+            stayers(base, cur);
+        }
+    } else if (no) {
+        GcPos B = load(WAVE), C = load(2 * WAVE);
+        for (uint32_t base = 0;; base += 3 * WAVE) {
+            stayers(base, A);
+            if (base + WAVE >= no) break;
+            A = load(base + 3 * WAVE);
+            stayers(base + WAVE, B);
+            if (base + 2 * WAVE >= no) break;
+            B = load(base + 4 * WAVE);
+            stayers(base + 2 * WAVE, C);
+            if (base + 3 * WAVE >= no) break;
+            C = load(base + 5 * WAVE);
+        }
     }
     for (uint32_t base = 0; base < nb; base += WAVE) {
         const uint32_t r = base + lane;
@@ -1336,6 +1362,7 @@ __device__ __forceinline__ void gc_move_cell(const GatherArgs &a, uint32_t c, ui
 // The gather stored only the arrivals' positions: each particle's state is
 // written to the new buffer once, here.  The moments are gc_write_back's,
 // summed in the same order.
+template <bool DEEP>
 __device__ __forceinline__ void gc_write_back_move(const GatherArgs &a, uint32_t c, const GcMeta &m,
                                                    uint32_t n_low, const Member &M, const float *lx,
                                                    const float *ly, const float *lz, uint32_t *mark,
@@ -1376,8 +1403,8 @@ __device__ __forceinline__ void gc_write_back_move(const GatherArgs &a, uint32_t
     }
     int32_t per[3] = {0, 0, 0};
     uint32_t leavers = 0, nslow = 0;
-    if (in_lds) gc_move_cell<true>(a, c, ns, os, no, nb, n_low, n_stay, M, V, lx, ly, lz, A, per, leavers, nslow);
-    else gc_move_cell<false>(a, c, ns, os, no, nb, n_low, n_stay, M, V, lx, ly, lz, A, per, leavers, nslow);
+    if (in_lds) gc_move_cell<true, DEEP>(a, c, ns, os, no, nb, n_low, n_stay, M, V, lx, ly, lz, A, per, leavers, nslow);
+    else gc_move_cell<false, DEEP>(a, c, ns, os, no, nb, n_low, n_stay, M, V, lx, ly, lz, A, per, leavers, nslow);
     const int32_t p0 = wave_sum(per[0]), p1 = wave_sum(per[1]), p2 = wave_sum(per[2]);
     if (lane == 0) {
         a.leave[c] = leavers;
@@ -1459,7 +1486,7 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
         if (MOVE) {
             // the move's registers are not shared with the next cell's batch:
             // that one is loaded afterwards
-            if (write_back) gc_write_back_move(a, c, mc, n_low, M, lx, ly, lz, mark, K);
+            if (write_back) gc_write_back_move<(GC_BATCH > 8)>(a, c, mc, n_low, M, lx, ly, lz, mark, K);
             else if (threadIdx.x == 0) { a.leave[c] = 0; a.slow_count[c] = 0; }
         }
         // the next cell: its first batch and keys go out before this cell's
