@@ -527,8 +527,8 @@ static constexpr int GATHER_BATCH = 8;      // chunks of 64 particles per round 
 // batch is in registers during a cell's write-back.
 static constexpr int GC_BATCH_SMALL = 5, GC_BATCH_LARGE = 11;
 // With the next step's move in the pass (gc_write_back_move) the batch holds
-// no positions: 5 (96 VGPRs) and 14 (160 VGPRs, with the move's stayers two
-// chunks ahead).
+// no positions: 5 (96 VGPRs) and 14 (164 VGPRs, with the move's stayers two
+// chunks ahead and its arrivals loaded before the tile).
 static constexpr int GC_BATCH_LARGE_MOVE = 14;
 
 __global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
@@ -1293,15 +1293,16 @@ __device__ __forceinline__ void gc_move_one(const GatherArgs &a, uint32_t c, uin
 // its load), the gather's classification repeated to place them (DEEP;
 // else one chunk ahead, for the small-batch variant's registers); A holds
 // the first chunk, loaded by the caller.  Arrivals, slots [0, n_low) and
-// [n_low + n_stay, n), from the new buffer, where the gather stored them.
+// [n_low + n_stay, n), from the new buffer, where the gather stored them
+// (DEEP: R holds the first chunk, loaded by the caller).
 // IN_LDS: the velocities' source (a merged path would make the compiler wait
 // for every load in flight, the prefetch included, before the first use).
 template <bool IN_LDS, bool DEEP>
 __device__ __forceinline__ void gc_move_cell(const GatherArgs &a, uint32_t c, uint32_t ns, uint32_t os,
                                              uint32_t no, uint32_t nb, uint32_t n_low, uint32_t n_stay,
                                              const Member &M, const VoxelView &V, const float *lx,
-                                             const float *ly, const float *lz, GcPos A, int32_t *per,
-                                             uint32_t &leavers, uint32_t &nslow) {
+                                             const float *ly, const float *lz, GcPos A, GcPos R,
+                                             int32_t *per, uint32_t &leavers, uint32_t &nslow) {
     const uint32_t lane = threadIdx.x;
     uint32_t run = 0;
     auto stayers = [&](uint32_t base, const GcPos &cur) {
@@ -1345,7 +1346,7 @@ __device__ __forceinline__ void gc_move_cell(const GatherArgs &a, uint32_t c, ui
         const uint32_t r = base + lane;
         const bool valid = r < nb;
         const uint32_t L = valid ? (r < n_low ? r : n_stay + r) : 0u;
-        const GcPos cur = gc_load_pos(a.ox2, a.oy2, a.oz2, a.vox2, ns + L);
+        const GcPos cur = DEEP && base == 0 ? R : gc_load_pos(a.ox2, a.oy2, a.oz2, a.vox2, ns + L);
         gc_move_one<IN_LDS>(a, c, ns, L, valid, M, V, lx, ly, lz, cur, per, leavers, nslow);
     }
 }
@@ -1373,9 +1374,14 @@ __device__ __forceinline__ void gc_write_back_move(const GatherArgs &a, uint32_t
     const uint32_t nb = m.be - m.bs, n_stay = n - nb;
     const bool in_lds = n <= K;
     const bool active = m.coeff > 0.0;
-    // the first stayer chunk goes out before the tile load
-    GcPos A{0.0f, 0.0f, 0.0f, 0u};
+    // the first stayer chunk and (DEEP) the first arrival chunk go out
+    // before the tile load
+    GcPos A{0.0f, 0.0f, 0.0f, 0u}, R{0.0f, 0.0f, 0.0f, 0u};
     if (no) A = gc_load_pos(a.ox, a.oy, a.oz, a.vox, os + (lane < no ? lane : no - 1));
+    if (DEEP && nb) {
+        const uint32_t L = lane < nb ? (lane < n_low ? lane : n_stay + lane) : 0u;
+        R = gc_load_pos(a.ox2, a.oy2, a.oz2, a.vox2, ns + L);
+    }
     // the cell's voxel tile in the mark table's LDS (free after the NTC loop)
     uint8_t *tile = (uint8_t *)mark;
     int cx, cy, cz;
@@ -1403,8 +1409,10 @@ __device__ __forceinline__ void gc_write_back_move(const GatherArgs &a, uint32_t
     }
     int32_t per[3] = {0, 0, 0};
     uint32_t leavers = 0, nslow = 0;
-    if (in_lds) gc_move_cell<true, DEEP>(a, c, ns, os, no, nb, n_low, n_stay, M, V, lx, ly, lz, A, per, leavers, nslow);
-    else gc_move_cell<false, DEEP>(a, c, ns, os, no, nb, n_low, n_stay, M, V, lx, ly, lz, A, per, leavers, nslow);
+    if (in_lds)
+        gc_move_cell<true, DEEP>(a, c, ns, os, no, nb, n_low, n_stay, M, V, lx, ly, lz, A, R, per, leavers, nslow);
+    else gc_move_cell<false, DEEP>(a, c, ns, os, no, nb, n_low, n_stay, M, V, lx, ly, lz, A, R, per, leavers,
+                                   nslow);
     const int32_t p0 = wave_sum(per[0]), p1 = wave_sum(per[1]), p2 = wave_sum(per[2]);
     if (lane == 0) {
         a.leave[c] = leavers;
