@@ -527,8 +527,9 @@ static constexpr int GATHER_BATCH = 8;      // chunks of 64 particles per round 
 // batch is in registers during a cell's write-back.
 static constexpr int GC_BATCH_SMALL = 5, GC_BATCH_LARGE = 11;
 // With the next step's move in the pass (gc_write_back_move) the batch holds
-// no positions: 5 (96 VGPRs) and 14 (164 VGPRs, with the move's stayers two
-// chunks ahead and its arrivals loaded before the tile).
+// no positions: 5 (96 VGPRs) and 14 (164 VGPRs, with the next cell's batch
+// in flight during the NTC loop, the move's stayers two chunks ahead and its
+// arrivals loaded before the tile).
 static constexpr int GC_BATCH_LARGE_MOVE = 14;
 
 __global__ __launch_bounds__(64) void k_gather(GatherArgs a) {
@@ -631,6 +632,10 @@ struct LdsStore {
     __device__ __forceinline__ float ly(uint32_t i) const { return y[i]; }
     __device__ __forceinline__ float lz(uint32_t i) const { return z[i]; }
     __device__ __forceinline__ void st(uint32_t i, float a, float b, float c) const { x[i] = a; y[i] = b; z[i] = c; }
+    // one wave per block, whose LDS accesses complete in order: a wavefront
+    // fence orders them (a block barrier would also wait for the wave's
+    // global loads in flight)
+    __device__ __forceinline__ static void sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 };
 
 struct GlobalStore {
@@ -647,6 +652,7 @@ struct GlobalStore {
     __device__ __forceinline__ void st(uint32_t i, float a, float b, float c) const {
         stf(x + i, a); stf(y + i, b); stf(z + i, c);
     }
+    __device__ __forceinline__ static void sync() { __syncthreads(); }
 };
 
 // Cell::collide (cell.cpp:75-111) for one cell, candidates in their
@@ -721,7 +727,7 @@ __device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint3
         pending = false;
         if (accept) { atomicMin(&mark[h0], k); atomicMin(&mark[h1], k); }
         for (;;) {
-            __syncthreads();
+            Store::sync();
             const bool dep = valid && (mark[h0] < k || mark[h1] < k);
             pending = pending || dep;
             const bool add = pending && !marked;
@@ -732,7 +738,7 @@ __device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint3
         if (commit && (double)vrel > crm) crm = (double)vrel;
         const bool scatter = commit && accept;
         acc += (uint32_t)__popcll(__ballot(scatter));
-        __syncthreads();
+        Store::sync();
         if (scatter) {
             const float cos_th = 1.0f - 2.0f * u01f(zc);
             const float sin_th = sqrtf(1.0f - cos_th * cos_th);
@@ -751,7 +757,7 @@ __device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint3
             S.st(i1, cmx - 0.5f * vrx, cmy - 0.5f * vry, cmz - 0.5f * vrz);
         }
         if (marked) { mark[h0] = NONE; mark[h1] = NONE; }
-        __syncthreads();
+        Store::sync();
     }
     crm_out = wave_max(crm);
     acc_out = acc;
@@ -1028,11 +1034,12 @@ __device__ __forceinline__ void gc_load_ahead(const GatherArgs &a, const GcMeta 
     P.key0 = threadIdx.x < nb ? a.bucket[m.bs + threadIdx.x] : 0xffffffffu;
 }
 
-// Gather (the first batch and the keys already loaded in P) and collide one
-// cell; returns whether its write-back (gc_write_back) is due and sets
-// n_low, the number of arrivals placed before the stayers.
+// Gather one cell (the first batch and the keys already loaded in P);
+// returns whether the cell has anything to do (gc_collide and a write-back
+// when it holds particles) and sets n_low, the number of arrivals placed
+// before the stayers.
 template <int GC_BATCH, bool MOVE>
-__device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t c, const GcMeta &m,
+__device__ __forceinline__ bool gc_gather(const GatherArgs &a, uint32_t c, const GcMeta &m,
                                                   const Member &M, GcAhead<GC_BATCH, !MOVE> &P, float *lx, float *ly,
                                                   float *lz, uint32_t *mark, uint32_t K, uint32_t &n_low_out) {
     const uint32_t lane = threadIdx.x;
@@ -1160,11 +1167,21 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
     }
     for (uint32_t i = lane; i < MARKS; i += WAVE) mark[i] = 0xffffffffu;
     __syncthreads();
-
-    // ---- collide (k_collide)
     n_low_out = n_low;
+    return true;
+}
+
+// Collide a gathered cell (k_collide): the NTC loop on its velocities in LDS
+// (or, above the LDS capacity, in the new buffer) and the cell's statistics.
+__device__ __forceinline__ void gc_collide(const GatherArgs &a, uint32_t c, const GcMeta &m, const Member &M,
+                                           float *lx, float *ly, float *lz, uint32_t *mark, uint32_t K) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t ns = m.ns, n = m.ne - m.ns;
+    const double coeff = m.coeff;
+    const bool active = coeff > 0.0;
+    const bool in_lds = n <= K;
     if (active && a.do_collide) {
-        const uint32_t step = *a.step;
+        const uint32_t step = ld_const(a.step);    // scalar: no wait on the vector loads in flight
         const double vrm = m.vrm;
         const double select = coeff * (double)n * (double)((int)n - 1) * vrm + m.rest;
         const int pairs = (int)round(select);
@@ -1188,7 +1205,6 @@ __device__ __forceinline__ bool gc_gather_collide(const GatherArgs &a, uint32_t 
             a.stat_acc[c] = (int32_t)acc;
         }
     }
-    return n > 0;
 }
 
 // Velocities of a gathered and collided cell back to the new buffer, and
@@ -1489,28 +1505,36 @@ __global__ __launch_bounds__(64) void k_gather_collide(GatherArgs a) {
             member = mi;
         }
         uint32_t n_low;
-        const bool write_back = gc_gather_collide<GC_BATCH, MOVE>(a, c, m, M, P, lx, ly, lz, mark, K, n_low);
+        const bool go = gc_gather<GC_BATCH, MOVE>(a, c, m, M, P, lx, ly, lz, mark, K, n_low);
         const GcMeta mc = m;
+        const bool write_back = go && mc.ne > mc.ns;
+        // the next cell: its first batch and keys go out before this cell's
+        // NTC loop (EARLY, with the move fused in: the loop waits on no
+        // global load, so they arrive meanwhile; the small-batch variant
+        // keeps its registers and loads them after the move) or before its
+        // write-back stores (waiting for them then does not wait for those)
+        constexpr bool EARLY = MOVE && GC_BATCH > 8;
+        auto next_cell = [&]() {
+            if (++j == a.chunk) {
+                cur = nxt;
+                q0 = q1;
+                q1 = grab_chunk(a.queue);
+                nxt = load_gc_batch(a, q1);
+                j = 0;
+            }
+            c64 = (uint64_t)q0 * a.chunk + j;
+            if (c64 < a.ncells) {
+                DSMC_GC_META(j);
+                gc_load_ahead<GC_BATCH, !MOVE>(a, m, P);
+            }
+        };
+        if (EARLY) next_cell();
+        if (go) gc_collide(a, c, mc, M, lx, ly, lz, mark, K);
         if (MOVE) {
-            // the move's registers are not shared with the next cell's batch:
-            // that one is loaded afterwards
             if (write_back) gc_write_back_move<(GC_BATCH > 8)>(a, c, mc, n_low, M, lx, ly, lz, mark, K);
             else if (threadIdx.x == 0) { a.leave[c] = 0; a.slow_count[c] = 0; }
         }
-        // the next cell: its first batch and keys go out before this cell's
-        // write-back stores, so waiting for them does not wait for those
-        if (++j == a.chunk) {
-            cur = nxt;
-            q0 = q1;
-            q1 = grab_chunk(a.queue);
-            nxt = load_gc_batch(a, q1);
-            j = 0;
-        }
-        c64 = (uint64_t)q0 * a.chunk + j;
-        if (c64 < a.ncells) {
-            DSMC_GC_META(j);
-            gc_load_ahead<GC_BATCH, !MOVE>(a, m, P);
-        }
+        if (!EARLY) next_cell();
         if (!MOVE && write_back) gc_write_back(a, c, mc, lx, ly, lz, K);
     }
 #undef DSMC_GC_META
