@@ -265,7 +265,7 @@ enum {
                                  * phases or Cercignani-Lampis walls */
 };
 typedef struct {
-    int graph_steps;            /* chosen steps per captured graph (2..32) */
+    int graph_steps;            /* chosen steps per captured graph (2..64) */
     int collide_chunk;          /* chosen cells per collide queue entry (0: automatic) */
     double ms_per_step;         /* best measured configuration */
     double ms_per_step_default; /* the default configuration, same measurement */

@@ -1673,7 +1673,7 @@ dsmc_status dsmc_autotune(dsmc_handle *h, int steps_per_trial, dsmc_tuning *out)
     HIPCHK(h, hipEventCreate(&e1));
     struct Cand { int gs; uint32_t chunk; };
     std::vector<Cand> cands;
-    const int gsteps[] = {2, 4, 8, 16, 32};
+    const int gsteps[] = {2, 4, 8, 16, 32, 64};
     const uint32_t chunks[] = {0, 1, 2, 4, 8, 16};
     for (int gs : gsteps)
         for (uint32_t ch : chunks) cands.push_back({gs, ch});
