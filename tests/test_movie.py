@@ -128,26 +128,34 @@ def test_rows_track_particles_between_frames(gpu, tmp_path):
 
 @pytest.mark.gpu
 def test_movie_leaves_the_trajectory_unchanged(gpu, tmp_path):
-    """Graphs and eager steps, with the sort carrying ids and move-ahead off,
-    give the trajectory of a run without a movie, and the same frames; an
-    ensemble's rows come from the member holding each particle."""
+    """Graphs and eager steps, with the sort carrying ids, give the trajectory
+    of a run without a movie and the same frames whatever the launch options
+    (a movie sets moving ahead aside); an ensemble's rows come from the
+    member holding each particle."""
     dsmc = gpu
     w = K.world()
     ini = K.settings(w, 20000, gravity=0.5)
     runs = {}
-    for name, movie, graph in [("plain", False, True), ("graph", True, True), ("eager", True, False)]:
+    for name, movie, graph, ahead in [("plain", False, True, 1), ("graph", True, True, 1),
+                                      ("eager", True, False, 1), ("noahead", True, False, 0)]:
         g = dsmc.System(ini, w)
         g.initialize()
         g.set_option(dsmc.OPT_USE_GRAPH, int(graph))
+        g.set_option(dsmc.OPT_MOVE_AHEAD, ahead)
+        if graph:
+            g.set_option(dsmc.OPT_GRAPH_STEPS, 4)
         if movie:
-            g.set_movie(tmp_path / f"{name}.bin", 2, 700)
+            g.set_movie(tmp_path / f"{name}.bin", 4, 700)
         g.step(12)
         runs[name] = g.state()
         g.close()
-    assert np.array_equal(runs["graph"], runs["plain"]) and np.array_equal(runs["eager"], runs["plain"])
-    a = M.read_frames(open(tmp_path / "graph.bin", "rb").read())
-    b = M.read_frames(open(tmp_path / "eager.bin", "rb").read())
-    assert len(a) == 6 and all(np.array_equal(x, y) for x, y in zip(a, b))
+    for name in ("graph", "eager", "noahead"):
+        assert np.array_equal(runs[name], runs["plain"]), name
+    ref = M.read_frames(open(tmp_path / "noahead.bin", "rb").read())
+    assert len(ref) == 3
+    for name in ("graph", "eager"):
+        got = M.read_frames(open(tmp_path / f"{name}.bin", "rb").read())
+        assert len(got) == 3 and all(np.array_equal(x, y) for x, y in zip(got, ref)), name
 
     w2 = K.world(n=32, r=3.0, phi=0.7, seed=4)
     ini2 = K.settings(w2, 15000, seed=9)
