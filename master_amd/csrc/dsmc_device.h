@@ -501,7 +501,8 @@ __device__ __forceinline__ bool move_fast(const Member &M, const VoxelView &V, i
     advance1(jx, ox, ax * M.inv_h[0], dt);
     advance1(jy, oy, ay * M.inv_h[1], dt);
     advance1(jz, oz, az * M.inv_h[2], dt);
-    const bool fast = (V.tile_at(ix, iy, iz) == V_EMPTY) & (V.tile_at(jx, jy, jz) == V_EMPTY);
+    // start and landing voxel both empty (V_EMPTY is 0), without a branch
+    const bool fast = (V.tile_at(ix, iy, iz) | V.tile_at(jx, jy, jz)) == V_EMPTY;
     // tile voxels lie within one voxel of the box: one wrap suffices
     const int wx = (jx >= M.n[0]) - (jx < 0), wy = (jy >= M.n[1]) - (jy < 0), wz = (jz >= M.n[2]) - (jz < 0);
     cnt.periodic[0] += fast ? wx : 0;

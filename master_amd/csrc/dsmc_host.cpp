@@ -218,7 +218,7 @@ dsmc_status build_member(dsmc_handle *h, const dsmc_settings &s, const dsmc_worl
         d.inv_b[a] = inv;
         // the kernels' cell index (int)((float)i * inv) must be i / b for
         // every voxel index of the axis: check them all
-        for (int i = 0; i < m.n[a]; i++) {
+        for (int i = 0; i < (int)m.n[a]; i++) {
             volatile float prod = (float)i * inv;
             if ((int)prod != i / b[a]) return fail(h, DSMC_ERR_CONFIG, "cell index by reciprocal not exact");
         }

@@ -674,7 +674,6 @@ template <class Store>
 __device__ __forceinline__ void ntc_cell(uint32_t key_lo, uint32_t key_hi, uint32_t cell_local, uint32_t step,
                                          uint32_t n, int pairs, double vrm, const Store &S, uint32_t *mark,
                                          double &crm_out, uint32_t &acc_out) {
-    const uint32_t lane = threadIdx.x;
     const uint32_t NONE = 0xffffffffu;
     double crm = vrm;
     uint32_t acc = 0;
