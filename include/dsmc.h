@@ -261,8 +261,8 @@ enum {
                                  * 512); the rest take the overflow pass */
     DSMC_OPT_MOVE_AHEAD = 11    /* 1 (default): inside one dsmc_step call the fused pass of a step
                                  * also moves for the next step (System::step's move follows its
-                                 * collide); off with recording, totals, statistics, partial
-                                 * phases or Cercignani-Lampis walls */
+                                 * collide); off with recording, totals, a movie or partial
+                                 * phases (statistics samples and movie frames end a chain) */
 };
 typedef struct {
     int graph_steps;            /* chosen steps per captured graph (2..64) */

@@ -610,8 +610,6 @@ void free_stats(dsmc_handle *h) {
 bool move_ahead_ok(const dsmc_handle *h) {
     if (!h->move_ahead || !h->fused || h->phases != 7 || h->record || h->totals || h->movie_every)
         return false;
-    for (auto &m : h->mem)
-        if (m.dev.wall_model == DSMC_WALL_CERCIGNANI_LAMPIS) return false;
     return true;
 }
 
@@ -638,6 +636,7 @@ MoveArgs slow_args(dsmc_handle *h) {
     m.step_offset = 1;
     m.slow_idx = h->d_slow_idx;
     m.slow_count = h->d_slow_cnt;
+    for (auto &mb : h->mem) m.any_cl |= mb.dev.wall_model == DSMC_WALL_CERCIGNANI_LAMPIS;
     return m;
 }
 

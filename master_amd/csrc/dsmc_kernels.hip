@@ -306,7 +306,9 @@ __global__ __launch_bounds__(64) void k_move_overflow(MoveArgs a) {
 // time with move_general from their post-collision state, exactly as
 // k_move's deferred loop would; leavers are appended to their cell's mover
 // list behind the fast path's (slots from an LDS counter per cell: the
-// order of a mover list is free, the gather ranks arrivals by index).
+// order of a mover list is free, the gather ranks arrivals by index).  CL:
+// some member has Cercignani-Lampis walls (chosen on the host, as k_move).
+template <bool CL>
 __global__ __launch_bounds__(64) void k_move_slow(MoveArgs a) {
     __shared__ uint32_t qstart[WAVE + 1], lslot[WAVE];
     const uint32_t lane = threadIdx.x;
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(64) void k_move_slow(MoveArgs a) {
             int ix, iy, iz;
             unpack_vox(at_off(a.vox, qo), ix, iy, iz);
             uint32_t nhit = 0;
-            (void)move_general<false>(kargs, mi, V, step, q - M.pfirst, ix, iy, iz, qox, qoy, qoz, qvx, qvy, qvz,
+            (void)move_general<CL>(kargs, mi, V, step, q - M.pfirst, ix, iy, iz, qox, qoy, qoz, qvx, qvy, qvz,
                                       cnt, nhit);
             at_off(a.ox, qo) = qox; at_off(a.oy, qo) = qoy; at_off(a.oz, qo) = qoz;
             at_off(a.vx, qo) = qvx; at_off(a.vy, qo) = qvy; at_off(a.vz, qo) = qvz;
@@ -1818,7 +1820,8 @@ hipError_t launch_gather_collide(const GatherArgs &a, uint32_t ncells, uint32_t 
 
 hipError_t launch_move_slow(const MoveArgs &a, uint32_t ncells, hipStream_t s) {
     if (ncells == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_move_slow, dim3((ncells + WAVE - 1) / WAVE), dim3(64), 0, s, a);
+    if (a.any_cl) hipLaunchKernelGGL(k_move_slow<true>, dim3((ncells + WAVE - 1) / WAVE), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(k_move_slow<false>, dim3((ncells + WAVE - 1) / WAVE), dim3(64), 0, s, a);
     LAUNCH_CHECK();
     return hipSuccess;
 }

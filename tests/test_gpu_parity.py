@@ -208,6 +208,31 @@ def test_fields_against_fp64_oracle(gpu):
     assert max(errs[50]) < 0.15
 
 
+def test_cercignani_lampis_moves_ahead(gpu):
+    """Cercignani-Lampis walls on the move-ahead path (their wall path in
+    k_move_slow, the draws keyed by step and particle) give the trajectory of
+    separate move passes bit for bit, eager and graphed."""
+    w = K.world()
+    ini = K.settings(w, 100_000, wall="cercignani_lampis", gravity=0.05)
+    ini["alpha_n"], ini["alpha_t"] = 0.6, 0.4
+    runs = []
+    for ahead, graph in ((0, 0), (1, 0), (1, 1)):
+        g = K.gpu_system(ini, w)
+        g.initialize()
+        g.set_sampling(1)
+        g.set_option(K.dsmc.OPT_MOVE_AHEAD, ahead)
+        g.set_option(K.dsmc.OPT_USE_GRAPH, graph)
+        g.step(12)
+        c = g.counters()
+        assert c["errors"] == 0
+        runs.append((g.state(), g.moments()[0], c["wall_collisions"], c["collisions"]))
+        g.close()
+    assert runs[0][2] > 1000
+    for r in runs[1:]:
+        assert np.array_equal(r[0], runs[0][0]) and np.array_equal(r[1], runs[0][1])
+        assert r[2:] == runs[0][2:]
+
+
 def test_collision_conservation(gpu):
     ini, w, g, o, st0 = _init_pair(gpu)
     g.set_option(K.dsmc.OPT_TOTALS, 1)
