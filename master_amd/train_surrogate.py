@@ -40,7 +40,7 @@ def main():
     ap.add_argument("--eval-steps", type=int, default=2000)
     ap.add_argument("--window", type=int, default=200)
     ap.add_argument("--replicas", type=int, default=8)
-    ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out"))
+    ap.add_argument("--out", default=os.path.join(ROOT, "surrogate_out"))
     ap.add_argument("--params", default=None, help="evaluate these parameters (.npy) instead of training")
     ap.add_argument("--synthetic", type=int, default=0,
                     help="train on this many random fields for --epochs, no DSMC and no evaluation "

@@ -119,7 +119,10 @@ def test_invalid_configuration_is_an_error_not_an_exit():
         dsmc.System(ini, w)
 
 
-def test_config_presets_reproduce_survey_numbers():
+def test_config_presets_reproduce_the_derived_gas_numbers():
+    """32^3 box, Kn 0.1 against the voxel edge, 200k particles: number
+    density 1/(sqrt(2) pi d^2 lambda), dt = 0.2 lambda / cbar (reduced) and
+    Fn from N = n V phi / Fn, as derived from the reference's units."""
     w = B.packed_spheres(32, 4.0, 0.6, 1)
     ini = C.dsmc_settings(32, w["porosity"], 200_000, 0.1, 8)
     assert float(ini["density"]) == pytest.approx(5.3768e26, rel=1e-4)

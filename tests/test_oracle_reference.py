@@ -73,8 +73,9 @@ def test_packed_spheres_bit_exact(golden, name):
         assert np.array_equal(frames, golden["g32_boundary_frames"])
 
 
-def test_packed_spheres_porosity_matches_survey(golden):
-    # the generator overshoots the 0.6 target: 0.5583 for seed 1
+def test_packed_spheres_porosity_overshoots_like_the_reference(golden):
+    # the reference generator mirrors one octant and overshoots the 0.6
+    # target (complexgeometry.cpp:446-511): 0.5583 for seed 1
     assert float(golden["g32_porosity"]) == pytest.approx(0.5583, abs=1e-4)
 
 
